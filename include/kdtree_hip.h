@@ -1,0 +1,163 @@
+/*
+ * kdtree_hip.h -- C ABI of the MI355X (gfx950) kD-tree Barnes-Hut step.
+ *
+ * Drop-in boundary for the reference's Rust crate rust_kdtree_nbody
+ * (Parallel/RustVersion/src/lib.rs:1-3).  The reference exposes a Rust library
+ * API and no FFI; every entry point below names the reference function whose
+ * work it replaces.  Plain C types only: host pointers, int64 sizes, int status.
+ *
+ * Conventions
+ *  - Every function returns KD_OK (0) or a negative KD_ERR_* code; the message
+ *    for the last failure on the calling thread is kd_last_error().
+ *  - A kd_ctx owns the device buffers for up to cap_n bodies and one HIP
+ *    stream.  One context must not be used from two threads at once; distinct
+ *    contexts are independent.
+ *  - Bodies are passed as structure-of-arrays in the caller's (original)
+ *    order; every result comes back in that order.
+ *  - Host arrays are copied; device-pointer entry points (suffix _device) read
+ *    device memory that the caller keeps alive for the duration of the call.
+ */
+#ifndef KDTREE_HIP_H
+#define KDTREE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KD_OK 0
+#define KD_ERR_ARG (-1)        /* bad argument (null pointer, n out of range, ...) */
+#define KD_ERR_CAPACITY (-2)   /* n above the context capacity / output too small */
+#define KD_ERR_HIP (-3)        /* a HIP runtime call failed (message has details) */
+#define KD_ERR_NOMEM (-4)      /* device or host allocation failed */
+#define KD_ERR_STATE (-5)      /* call out of order (e.g. step before upload) */
+#define KD_ERR_NONFINITE (-6)  /* NaN or Inf in positions or masses */
+#define KD_ERR_NODEVICE (-7)   /* no usable gfx950 device */
+
+/* Arithmetic of the force traversal. */
+#define KD_PREC_FP64 0         /* reference arithmetic: FP64, sqrt then divide, no FMA */
+#define KD_PREC_MIXED 1        /* FP32 far-field and P2P, compensated FP32 sums,
+                                  FP64 state, integrator and node moments */
+#define KD_PREC_MIXED_D64 2    /* as MIXED but coordinate differences taken in FP64 */
+
+/* Node-vector layouts of the reference. */
+#define KD_LAYOUT_COMPACT 0    /* Sequential/RustVersion/src/array_kd_tree.rs:128-134 */
+#define KD_LAYOUT_PADDED 1     /* Parallel/RustVersion/src/array_kd_tree.rs:565-581 */
+
+typedef struct kd_params {
+  double theta;      /* opening angle; THETA = 0.3 at array_kd_tree.rs:15 */
+  double dt;         /* time step; dt = 1e-3 at Parallel/RustVersion/src/main.rs:23 */
+  int32_t leaf_size; /* MAX_PARTS: 7 (Sequential :6) or 8 (Parallel :14); 2..32 */
+  int32_t precision; /* KD_PREC_* */
+  int32_t count_interactions; /* nonzero: kernels also count tests/M2P/P2P */
+  int32_t reserved;
+} kd_params;
+
+typedef struct kd_ctx kd_ctx;
+
+/* Flat view of a node vector in one of the reference layouts
+ * (KDTree enum, Sequential/RustVersion/src/array_kd_tree.rs:10-26).  Leaves
+ * list their members as perm[first .. first+num_parts); perm holds original
+ * body indices in particle order (leaves contiguous, depth-first, members in
+ * ascending original index).  Unused padded slots are leaves with
+ * num_parts == 0, as allocate_node_vec fills them. */
+typedef struct kd_tree_view {
+  int64_t n_nodes;     /* in: capacity of every node array; out: nodes written */
+  int32_t *is_leaf;
+  int32_t *split_dim;
+  int64_t *num_parts;
+  int64_t *first;
+  double *split_val;
+  double *m;
+  double *cmx, *cmy, *cmz;
+  double *size;
+  int64_t *left, *right;
+  int64_t *perm;       /* n entries */
+} kd_tree_view;
+
+const char *kd_last_error(void);
+/* Library version string and the compiled target ("gfx950"). */
+const char *kd_version(void);
+
+/* theta 0.3, dt 1e-3, leaf 7, FP64, no counting. */
+int kd_default_params(kd_params *out);
+
+/* Context with capacity for cap_n bodies on the current HIP device.
+ * Replaces the per-call allocations of simple_sim (acc, tree, indices:
+ * Parallel/RustVersion/src/array_kd_tree.rs:624-630). */
+int kd_create(int64_t cap_n, const kd_params *params, kd_ctx **out);
+int kd_destroy(kd_ctx *ctx);
+int kd_set_params(kd_ctx *ctx, const kd_params *params);
+int kd_get_params(const kd_ctx *ctx, kd_params *out);
+
+/* Copy n bodies in (the caller's Vec<Particle>, array_particle.rs:3-8).
+ * -0.0 coordinates are stored as +0.0; NaN/Inf positions or masses and
+ * non-positive masses are rejected with KD_ERR_NONFINITE / KD_ERR_ARG. */
+int kd_upload(kd_ctx *ctx, int64_t n, const double *x, const double *y,
+              const double *z, const double *vx, const double *vy,
+              const double *vz, const double *m);
+int kd_upload_device(kd_ctx *ctx, int64_t n, const double *x, const double *y,
+                     const double *z, const double *vx, const double *vy,
+                     const double *vz, const double *m);
+/* Copy the current state out (any pointer may be NULL to skip it). */
+int kd_download(kd_ctx *ctx, double *x, double *y, double *z, double *vx,
+                double *vy, double *vz);
+int kd_download_device(kd_ctx *ctx, double *x, double *y, double *z,
+                       double *vx, double *vy, double *vz);
+int64_t kd_size(const kd_ctx *ctx);
+
+/* Build the tree of the current positions: build_tree
+ * (Sequential/RustVersion/src/array_kd_tree.rs:49) / build_tree_par4
+ * (Parallel/RustVersion/src/array_kd_tree.rs:515), including node moments. */
+int kd_build(kd_ctx *ctx);
+
+/* Export the last built tree in a reference layout (allocate_node_vec +
+ * build_tree's node vector, array_kd_tree.rs:41-46 / :55-60).  If
+ * view->n_nodes is smaller than needed, returns KD_ERR_CAPACITY and stores
+ * the required count in view->n_nodes. */
+int kd_export_tree(kd_ctx *ctx, int32_t layout, kd_tree_view *view);
+/* Node-vector length the reference would allocate for n bodies. */
+int64_t kd_nodes_needed(int64_t n, int32_t leaf_size, int32_t layout);
+
+/* Build, then accelerations of every body: calc_accel(p, ..) for all p
+ * (array_kd_tree.rs:619 / Sequential :174), in original order. */
+int kd_accel(kd_ctx *ctx, double *ax, double *ay, double *az);
+int kd_accel_device(kd_ctx *ctx, double *ax, double *ay, double *az);
+
+/* steps x (build, accel, kick-drift v += dt*a; p += dt*v): simple_sim
+ * (Parallel/RustVersion/src/array_kd_tree.rs:623-664). */
+int kd_step(kd_ctx *ctx, int64_t steps);
+/* Wait for the context's stream. */
+int kd_sync(kd_ctx *ctx);
+
+/* Node tests, accepted far-field interactions and P2P pairs summed over the
+ * bodies of the last traversal (needs params.count_interactions). */
+int kd_counters(kd_ctx *ctx, int64_t out[3]);
+
+/* FP64 O(N) direct sum (calc_pp_accel form, array_particle.rs:67-76) for k
+ * sample bodies given by original index, over all bodies; abs_* (nullable)
+ * receive the per-component sum of |a_ij|. */
+int kd_direct_sum(kd_ctx *ctx, int64_t k, const int64_t *idx, double *ax,
+                  double *ay, double *az, double *absx, double *absy,
+                  double *absz);
+
+/* Synthetic initial conditions (host only, no device needed).
+ * kd_ic_ring: body 0 mass 1 at rest at the origin; bodies 1..n-1 mass 1e-14,
+ * r ~ U[1,2), phi ~ U[0,2pi), z ~ N(0, 1e-3), v = sqrt(1/r)(-sin phi, cos phi, 0).
+ * kd_ic_circular_orbits: circular_orbits(n) of array_particle.rs:19-44 (n+1
+ * bodies, d = 0.1 + 5i/n, angle U[0,6.28), planar) with a seeded stream.
+ * kd_ic_two_bodies: two_bodies() of array_particle.rs:10-17 (2 bodies).
+ * Random streams are splitmix64(seed); uniforms are (u >> 11) * 2^-53. */
+int kd_ic_ring(uint64_t seed, int64_t n, double *x, double *y, double *z,
+               double *vx, double *vy, double *vz, double *m);
+int kd_ic_circular_orbits(uint64_t seed, int64_t n, double *x, double *y,
+                          double *z, double *vx, double *vy, double *vz,
+                          double *m);
+int kd_ic_two_bodies(double *x, double *y, double *z, double *vx, double *vy,
+                     double *vz, double *m);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,691 @@
+// kd_api.hip -- C ABI of the gfx950 kD-tree Barnes-Hut step (kdtree_hip.h).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/kdtree_hip.h"
+#include "kd_internal.hpp"
+
+
+using namespace kd;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                             \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(e_ == hipErrorOutOfMemory ? KD_ERR_NOMEM : KD_ERR_HIP,          \
+                  "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                  __LINE__);                                                      \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+
+struct kd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  kd_params params{};
+  int64_t cap = 0;
+  int64_t n = 0;
+  bool has_state = false;
+  bool has_tree = false;
+  Shape shape;
+  int64_t slots_alloc = 0;
+
+  StateDev st[2]{};
+  int cur = 0;
+  BuildBufs b{};
+  TreeDev tree{};
+  double4 *pos64 = nullptr;
+  float4 *pos32 = nullptr;
+  double *acc[3] = {nullptr, nullptr, nullptr};
+  double *acc_o[3] = {nullptr, nullptr, nullptr};
+  uint32_t *inv = nullptr;
+  unsigned long long *counters = nullptr;
+  int *bad = nullptr;
+  std::vector<void *> allocs;
+  std::vector<void *> node_allocs;
+};
+
+template <typename T>
+static int dalloc(kd_ctx *c, std::vector<void *> &list, T **p, size_t count) {
+  void *q = nullptr;
+  size_t bytes = count * sizeof(T);
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess)
+    return fail(KD_ERR_NOMEM, "hipMalloc(%zu bytes) failed: %s", bytes, hipGetErrorString(e));
+  list.push_back(q);
+  *p = (T *)q;
+  return KD_OK;
+}
+
+#define ALLOC(list, ptr, count)                     \
+  do {                                              \
+    int rc_ = dalloc(c, list, &(ptr), (size_t)(count)); \
+    if (rc_) return rc_;                            \
+  } while (0)
+
+static int check_params(const kd_params *p) {
+  if (!p) return fail(KD_ERR_ARG, "params is NULL");
+  if (!(p->theta >= 0.0) || !std::isfinite(p->theta))
+    return fail(KD_ERR_ARG, "theta must be finite and >= 0 (got %g)", p->theta);
+  if (!std::isfinite(p->dt)) return fail(KD_ERR_ARG, "dt must be finite");
+  if (p->leaf_size < 2 || p->leaf_size > MAX_LEAF)
+    return fail(KD_ERR_ARG, "leaf_size must be in [2, %d] (got %d)", MAX_LEAF, p->leaf_size);
+  if (p->precision < KD_PREC_FP64 || p->precision > KD_PREC_MIXED_D64)
+    return fail(KD_ERR_ARG, "unknown precision %d", p->precision);
+  return KD_OK;
+}
+
+static int ensure_nodes(kd_ctx *c) {
+  Shape sh = make_shape(c->n, c->params.leaf_size);
+  c->shape = sh;
+  if (sh.slots > c->slots_alloc) {
+    for (void *p : c->node_allocs) (void)hipFree(p);
+    c->node_allocs.clear();
+    c->slots_alloc = 0;
+    int64_t s = sh.slots;
+    TreeDev &t = c->tree;
+    ALLOC(c->node_allocs, t.seg_start, s);
+    ALLOC(c->node_allocs, t.seg_count, s);
+    ALLOC(c->node_allocs, t.split_dim, s);
+    ALLOC(c->node_allocs, t.split_val, s);
+    ALLOC(c->node_allocs, t.size, s);
+    ALLOC(c->node_allocs, t.mm, s);
+    ALLOC(c->node_allocs, t.mx, s);
+    ALLOC(c->node_allocs, t.my, s);
+    ALLOC(c->node_allocs, t.mz, s);
+    ALLOC(c->node_allocs, t.rec64, s);
+    ALLOC(c->node_allocs, t.rec32, s);
+    ALLOC(c->node_allocs, t.recd, s);
+    c->slots_alloc = s;
+  }
+  HIP_TRY(launch_shape(c->tree, sh, c->stream));
+  return KD_OK;
+}
+
+extern "C" {
+
+const char *kd_last_error(void) { return g_err.c_str(); }
+
+const char *kd_version(void) { return "kdtree_hip 0.1 gfx950"; }
+
+int kd_default_params(kd_params *out) {
+  if (!out) return fail(KD_ERR_ARG, "out is NULL");
+  memset(out, 0, sizeof(*out));
+  out->theta = 0.3;
+  out->dt = 1e-3;
+  out->leaf_size = 7;
+  out->precision = KD_PREC_FP64;
+  out->count_interactions = 0;
+  return KD_OK;
+}
+
+int64_t kd_nodes_needed(int64_t n, int32_t leaf, int32_t layout) {
+  if (n < 0 || leaf < 2) return -1;
+  int64_t half = leaf / 2;
+  if (layout == KD_LAYOUT_PADDED) {
+    if (n <= leaf) return 1;
+    int64_t min_leaves = n / half;
+    int p = 0;
+    while (((int64_t)1 << p) < min_leaves) ++p;
+    return 2 * ((int64_t)1 << p) - 1;
+  }
+  return 2 * (n / half + 1);
+}
+
+int kd_destroy(kd_ctx *c) {
+  if (!c) return KD_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (void *p : c->allocs) (void)hipFree(p);
+  for (void *p : c->node_allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return KD_OK;
+}
+
+static int create_impl(kd_ctx *c, int64_t cap) {
+  HIP_TRY(hipGetDevice(&c->device));
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, c->device));
+  if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(KD_ERR_NODEVICE, "device %d is %s; this library is built for gfx950",
+                c->device, prop.gcnArchName);
+  HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  size_t N = (size_t)cap;
+  for (int i = 0; i < 2; ++i) {
+    StateDev &s = c->st[i];
+    ALLOC(c->allocs, s.x, N);
+    ALLOC(c->allocs, s.y, N);
+    ALLOC(c->allocs, s.z, N);
+    ALLOC(c->allocs, s.vx, N);
+    ALLOC(c->allocs, s.vy, N);
+    ALLOC(c->allocs, s.vz, N);
+    ALLOC(c->allocs, s.m, N);
+    ALLOC(c->allocs, s.oid, N);
+  }
+  for (int d = 0; d < 3; ++d) {
+    ALLOC(c->allocs, c->b.L[d], N);
+    ALLOC(c->allocs, c->b.Lb[d], N);
+    ALLOC(c->allocs, c->acc[d], N);
+    ALLOC(c->allocs, c->acc_o[d], N);
+  }
+  ALLOC(c->allocs, c->b.keys_in, N);
+  ALLOC(c->allocs, c->b.keys_out, N);
+  ALLOC(c->allocs, c->b.ids_in, N);
+  ALLOC(c->allocs, c->b.flags, N);
+  ALLOC(c->allocs, c->b.loc, N);
+  ALLOC(c->allocs, c->b.perm, N);
+  ALLOC(c->allocs, c->inv, N);
+  ALLOC(c->allocs, c->pos64, N);
+  ALLOC(c->allocs, c->pos32, N);
+  size_t max_seg = 2 * (N / SUB_MAX) + 64;
+  ALLOC(c->allocs, c->b.cnt, 3 * (N / LVL_TILE + max_seg + 64));
+  ALLOC(c->allocs, c->b.tieflag, max_seg);
+  c->b.sort_tmp_bytes = sort_temp_bytes((uint32_t)cap);
+  ALLOC(c->allocs, *(uint8_t **)&c->b.sort_tmp, c->b.sort_tmp_bytes);
+  ALLOC(c->allocs, c->counters, 4);
+  ALLOC(c->allocs, c->bad, 4);
+  return KD_OK;
+}
+
+int kd_create(int64_t cap_n, const kd_params *params, kd_ctx **out) {
+  if (!out) return fail(KD_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (cap_n < 1 || cap_n > 0x7fffffffLL)
+    return fail(KD_ERR_ARG, "capacity must be in [1, 2^31-1] (got %lld)", (long long)cap_n);
+  kd_params p;
+  if (params) {
+    p = *params;
+  } else {
+    kd_default_params(&p);
+  }
+  int rc = check_params(&p);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(KD_ERR_NODEVICE, "no HIP device visible");
+  kd_ctx *c = new kd_ctx();
+  c->params = p;
+  c->cap = cap_n;
+  rc = create_impl(c, cap_n);
+  if (rc) {
+    std::string msg = g_err;
+    kd_destroy(c);
+    g_err = msg;
+    return rc;
+  }
+  *out = c;
+  return KD_OK;
+}
+
+int kd_set_params(kd_ctx *c, const kd_params *p) {
+  if (!c) return fail(KD_ERR_ARG, "ctx is NULL");
+  int rc = check_params(p);
+  if (rc) return rc;
+  bool leaf_changed = p->leaf_size != c->params.leaf_size;
+  c->params = *p;
+  if (leaf_changed) c->has_tree = false;
+  if (leaf_changed && c->has_state) {
+    HIP_TRY(hipSetDevice(c->device));
+    rc = ensure_nodes(c);
+    if (rc) return rc;
+  }
+  return KD_OK;
+}
+
+int kd_get_params(const kd_ctx *c, kd_params *out) {
+  if (!c || !out) return fail(KD_ERR_ARG, "NULL argument");
+  *out = c->params;
+  return KD_OK;
+}
+
+int64_t kd_size(const kd_ctx *c) { return c ? c->n : -1; }
+
+static int upload_common(kd_ctx *c, int64_t n) {
+  StateDev &s = c->st[c->cur];
+  HIP_TRY(launch_iota(s.oid, (uint32_t)n, c->stream));
+  HIP_TRY(hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
+  HIP_TRY(launch_sanitize(s, (uint32_t)n, c->bad, c->stream));
+  int bad = 0;
+  HIP_TRY(hipMemcpyAsync(&bad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (bad) {
+    c->has_state = false;
+    return fail(KD_ERR_NONFINITE, "non-finite position, velocity or mass in upload");
+  }
+  c->n = n;
+  c->has_state = true;
+  c->has_tree = false;
+  return ensure_nodes(c);
+}
+
+static int upload_impl(kd_ctx *c, int64_t n, const double *const src[7], hipMemcpyKind kind) {
+  if (!c) return fail(KD_ERR_ARG, "ctx is NULL");
+  for (int i = 0; i < 7; ++i)
+    if (!src[i]) return fail(KD_ERR_ARG, "input array %d is NULL", i);
+  if (n < 1) return fail(KD_ERR_ARG, "n must be >= 1 (got %lld)", (long long)n);
+  if (n > c->cap)
+    return fail(KD_ERR_CAPACITY, "n = %lld exceeds the context capacity %lld", (long long)n,
+                (long long)c->cap);
+  HIP_TRY(hipSetDevice(c->device));
+  StateDev &s = c->st[c->cur];
+  double *dst[7] = {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m};
+  for (int i = 0; i < 7; ++i)
+    HIP_TRY(hipMemcpyAsync(dst[i], src[i], sizeof(double) * (size_t)n, kind, c->stream));
+  return upload_common(c, n);
+}
+
+int kd_upload(kd_ctx *c, int64_t n, const double *x, const double *y, const double *z,
+              const double *vx, const double *vy, const double *vz, const double *m) {
+  const double *src[7] = {x, y, z, vx, vy, vz, m};
+  return upload_impl(c, n, src, hipMemcpyHostToDevice);
+}
+
+int kd_upload_device(kd_ctx *c, int64_t n, const double *x, const double *y, const double *z,
+                     const double *vx, const double *vy, const double *vz, const double *m) {
+  const double *src[7] = {x, y, z, vx, vy, vz, m};
+  return upload_impl(c, n, src, hipMemcpyDeviceToDevice);
+}
+
+static int download_impl(kd_ctx *c, double *const out[6], hipMemcpyKind kind) {
+  if (!c) return fail(KD_ERR_ARG, "ctx is NULL");
+  if (!c->has_state) return fail(KD_ERR_STATE, "no bodies uploaded");
+  HIP_TRY(hipSetDevice(c->device));
+  StateDev &s = c->st[c->cur];
+  const double *src[6] = {s.x, s.y, s.z, s.vx, s.vy, s.vz};
+  uint32_t n = (uint32_t)c->n;
+  for (int i = 0; i < 6; i += 3) {
+    HIP_TRY(launch_to_original(s.oid, src[i], src[i + 1], src[i + 2], c->acc_o[0], c->acc_o[1],
+                               c->acc_o[2], n, c->stream));
+    for (int k = 0; k < 3; ++k)
+      if (out[i + k])
+        HIP_TRY(hipMemcpyAsync(out[i + k], c->acc_o[k], sizeof(double) * n, kind, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+  }
+  return KD_OK;
+}
+
+int kd_download(kd_ctx *c, double *x, double *y, double *z, double *vx, double *vy, double *vz) {
+  double *out[6] = {x, y, z, vx, vy, vz};
+  return download_impl(c, out, hipMemcpyDeviceToHost);
+}
+
+int kd_download_device(kd_ctx *c, double *x, double *y, double *z, double *vx, double *vy,
+                       double *vz) {
+  double *out[6] = {x, y, z, vx, vy, vz};
+  return download_impl(c, out, hipMemcpyDeviceToDevice);
+}
+
+static int build_impl(kd_ctx *c) {
+  int nxt = c->cur ^ 1;
+  HIP_TRY(launch_build(c->shape, c->st[c->cur], c->st[nxt], c->tree, c->b, c->pos64, c->pos32,
+                       c->stream));
+  c->cur = nxt;
+  c->has_tree = true;
+  return KD_OK;
+}
+
+static int traverse_impl(kd_ctx *c) {
+  bool count = c->params.count_interactions != 0;
+  if (count) HIP_TRY(hipMemsetAsync(c->counters, 0, 4 * sizeof(unsigned long long), c->stream));
+  HIP_TRY(launch_traverse(c->params.precision, count, c->tree, c->pos64, c->pos32,
+                          (uint32_t)c->n, c->params.theta, c->params.leaf_size, c->acc[0],
+                          c->acc[1], c->acc[2], c->counters, c->stream));
+  return KD_OK;
+}
+
+int kd_build(kd_ctx *c) {
+  if (!c) return fail(KD_ERR_ARG, "ctx is NULL");
+  if (!c->has_state) return fail(KD_ERR_STATE, "no bodies uploaded");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = build_impl(c);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return KD_OK;
+}
+
+static int accel_impl(kd_ctx *c, double *ax, double *ay, double *az, hipMemcpyKind kind) {
+  if (!c) return fail(KD_ERR_ARG, "ctx is NULL");
+  if (!c->has_state) return fail(KD_ERR_STATE, "no bodies uploaded");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = build_impl(c);
+  if (rc) return rc;
+  rc = traverse_impl(c);
+  if (rc) return rc;
+  uint32_t n = (uint32_t)c->n;
+  HIP_TRY(launch_to_original(c->st[c->cur].oid, c->acc[0], c->acc[1], c->acc[2], c->acc_o[0],
+                             c->acc_o[1], c->acc_o[2], n, c->stream));
+  double *out[3] = {ax, ay, az};
+  for (int k = 0; k < 3; ++k)
+    if (out[k])
+      HIP_TRY(hipMemcpyAsync(out[k], c->acc_o[k], sizeof(double) * n, kind, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return KD_OK;
+}
+
+int kd_accel(kd_ctx *c, double *ax, double *ay, double *az) {
+  return accel_impl(c, ax, ay, az, hipMemcpyDeviceToHost);
+}
+
+int kd_accel_device(kd_ctx *c, double *ax, double *ay, double *az) {
+  return accel_impl(c, ax, ay, az, hipMemcpyDeviceToDevice);
+}
+
+int kd_step(kd_ctx *c, int64_t steps) {
+  if (!c) return fail(KD_ERR_ARG, "ctx is NULL");
+  if (!c->has_state) return fail(KD_ERR_STATE, "no bodies uploaded");
+  if (steps < 0) return fail(KD_ERR_ARG, "steps must be >= 0");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
+  for (int64_t s = 0; s < steps; ++s) {
+    int rc = build_impl(c);
+    if (rc) return rc;
+    rc = traverse_impl(c);
+    if (rc) return rc;
+    HIP_TRY(launch_kick_drift(c->st[c->cur], c->acc[0], c->acc[1], c->acc[2], (uint32_t)c->n,
+                              c->params.dt, c->bad, c->stream));
+  }
+  int bad = 0;
+  HIP_TRY(hipMemcpyAsync(&bad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (bad) return fail(KD_ERR_NONFINITE, "a body reached a non-finite position");
+  return KD_OK;
+}
+
+int kd_sync(kd_ctx *c) {
+  if (!c) return fail(KD_ERR_ARG, "ctx is NULL");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return KD_OK;
+}
+
+int kd_counters(kd_ctx *c, int64_t out[3]) {
+  if (!c || !out) return fail(KD_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(c->device));
+  unsigned long long h[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < 3; ++i) out[i] = (int64_t)h[i];
+  return KD_OK;
+}
+
+int kd_direct_sum(kd_ctx *c, int64_t k, const int64_t *idx, double *ax, double *ay,
+                  double *az, double *absx, double *absy, double *absz) {
+  if (!c || !idx || !ax || !ay || !az) return fail(KD_ERR_ARG, "NULL argument");
+  if (!c->has_state) return fail(KD_ERR_STATE, "no bodies uploaded");
+  if (k < 1 || k > c->n) return fail(KD_ERR_ARG, "k must be in [1, n]");
+  for (int64_t i = 0; i < k; ++i)
+    if (idx[i] < 0 || idx[i] >= c->n)
+      return fail(KD_ERR_ARG, "sample index %lld out of range", (long long)idx[i]);
+  HIP_TRY(hipSetDevice(c->device));
+  uint32_t n = (uint32_t)c->n;
+  StateDev &s = c->st[c->cur];
+  HIP_TRY(launch_inverse(s.oid, c->inv, n, c->stream));
+  std::vector<uint32_t> inv(n);
+  HIP_TRY(hipMemcpyAsync(inv.data(), c->inv, sizeof(uint32_t) * n, hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<uint32_t> wpos(k);
+  for (int64_t i = 0; i < k; ++i) wpos[i] = inv[idx[i]];
+  uint32_t *dw = nullptr;
+  double *dout = nullptr;
+  HIP_TRY(hipMalloc(&dw, sizeof(uint32_t) * k));
+  hipError_t e = hipMalloc(&dout, sizeof(double) * 6 * k);
+  if (e != hipSuccess) {
+    (void)hipFree(dw);
+    return fail(KD_ERR_NOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+  }
+  std::vector<double> out(6 * k);
+  e = hipMemcpyAsync(dw, wpos.data(), sizeof(uint32_t) * k, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = launch_direct(s, n, dw, (uint32_t)k, dout, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(out.data(), dout, sizeof(double) * 6 * k, hipMemcpyDeviceToHost,
+                       c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dw);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return fail(KD_ERR_HIP, "direct sum failed: %s", hipGetErrorString(e));
+  for (int64_t i = 0; i < k; ++i) {
+    ax[i] = out[6 * i + 0];
+    ay[i] = out[6 * i + 1];
+    az[i] = out[6 * i + 2];
+    if (absx) absx[i] = out[6 * i + 3];
+    if (absy) absy[i] = out[6 * i + 4];
+    if (absz) absz[i] = out[6 * i + 5];
+  }
+  return KD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// export to the reference node-vector layouts
+// ---------------------------------------------------------------------------
+
+struct HostTree {
+  std::vector<uint32_t> start, count;
+  std::vector<uint8_t> sd;
+  std::vector<double> sv, size;
+  std::vector<NodeRec64> rec;
+  int leaf;
+};
+
+// Compact pre-order layout: left = cur+1, right = last node of left + 1
+// (Sequential/RustVersion/src/array_kd_tree.rs:128-134).  Returns the last
+// node index used.
+static int64_t emit_compact(const HostTree &t, uint32_t h, int64_t cur, kd_tree_view *v) {
+  uint32_t cnt = t.count[h];
+  if (cnt <= (uint32_t)t.leaf) {
+    v->is_leaf[cur] = 1;
+    v->num_parts[cur] = cnt;
+    v->first[cur] = t.start[h];
+    return cur;
+  }
+  int64_t left = emit_compact(t, 2 * h + 1, cur + 1, v);
+  int64_t right = emit_compact(t, 2 * h + 2, left + 1, v);
+  v->is_leaf[cur] = 0;
+  v->split_dim[cur] = t.sd[h];
+  v->split_val[cur] = t.sv[h];
+  v->size[cur] = t.size[h];
+  v->m[cur] = t.rec[h].m;
+  v->cmx[cur] = t.rec[h].cx;
+  v->cmy[cur] = t.rec[h].cy;
+  v->cmz[cur] = t.rec[h].cz;
+  v->left[cur] = cur + 1;
+  v->right[cur] = left + 1;
+  return right;
+}
+
+// Padded layout: right = cur + 1 + nodes_needed(floor(n/2))
+// (Parallel/RustVersion/src/array_kd_tree.rs:565-581).  Returns false when a
+// subtree overflows its slice (the reference panics there).
+static bool emit_padded(const HostTree &t, uint32_t h, int64_t cur, int64_t limit,
+                        kd_tree_view *v) {
+  if (cur >= limit) return false;
+  uint32_t cnt = t.count[h];
+  if (cnt <= (uint32_t)t.leaf) {
+    v->is_leaf[cur] = 1;
+    v->num_parts[cur] = cnt;
+    v->first[cur] = t.start[h];
+    return true;
+  }
+  int64_t nn = kd_nodes_needed(cnt / 2, t.leaf, KD_LAYOUT_PADDED);
+  if (nn > limit - cur - 1) return false;
+  if (!emit_padded(t, 2 * h + 1, cur + 1, cur + 1 + nn, v)) return false;
+  if (!emit_padded(t, 2 * h + 2, cur + 1 + nn, limit, v)) return false;
+  v->is_leaf[cur] = 0;
+  v->split_dim[cur] = t.sd[h];
+  v->split_val[cur] = t.sv[h];
+  v->size[cur] = t.size[h];
+  v->m[cur] = t.rec[h].m;
+  v->cmx[cur] = t.rec[h].cx;
+  v->cmy[cur] = t.rec[h].cy;
+  v->cmz[cur] = t.rec[h].cz;
+  v->left[cur] = cur + 1;
+  v->right[cur] = cur + 1 + nn;
+  return true;
+}
+
+static int64_t compact_used(int64_t n, int leaf) {
+  if (n <= leaf) return 1;
+  return 1 + compact_used(n / 2, leaf) + compact_used(n - n / 2, leaf);
+}
+
+int kd_export_tree(kd_ctx *c, int32_t layout, kd_tree_view *v) {
+  if (!c || !v) return fail(KD_ERR_ARG, "NULL argument");
+  if (!c->has_tree) return fail(KD_ERR_STATE, "no tree built");
+  if (layout != KD_LAYOUT_COMPACT && layout != KD_LAYOUT_PADDED)
+    return fail(KD_ERR_ARG, "unknown layout %d", layout);
+  const int leaf = c->params.leaf_size;
+  const int64_t n = c->n;
+  int64_t need;
+  if (layout == KD_LAYOUT_PADDED) {
+    need = kd_nodes_needed(n, leaf, layout);
+  } else {
+    need = std::max(kd_nodes_needed(n, leaf, layout), compact_used(n, leaf));
+  }
+  if (v->n_nodes < need) {
+    v->n_nodes = need;
+    return fail(KD_ERR_CAPACITY, "node arrays hold fewer than %lld nodes", (long long)need);
+  }
+  if (!v->is_leaf || !v->split_dim || !v->num_parts || !v->first || !v->split_val || !v->m ||
+      !v->cmx || !v->cmy || !v->cmz || !v->size || !v->left || !v->right || !v->perm)
+    return fail(KD_ERR_ARG, "a node array is NULL");
+  HIP_TRY(hipSetDevice(c->device));
+  const int64_t slots = c->shape.slots;
+  HostTree t;
+  t.leaf = leaf;
+  t.start.resize(slots);
+  t.count.resize(slots);
+  t.sd.resize(slots);
+  t.sv.resize(slots);
+  t.size.resize(slots);
+  t.rec.resize(slots);
+  std::vector<uint32_t> oid(n);
+  hipStream_t s = c->stream;
+  HIP_TRY(hipMemcpyAsync(t.start.data(), c->tree.seg_start, 4 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.count.data(), c->tree.seg_count, 4 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.sd.data(), c->tree.split_dim, slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.sv.data(), c->tree.split_val, 8 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.size.data(), c->tree.size, 8 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.rec.data(), c->tree.rec64, sizeof(NodeRec64) * slots,
+                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(oid.data(), c->st[c->cur].oid, 4 * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  for (int64_t i = 0; i < need; ++i) {
+    v->is_leaf[i] = 1;
+    v->split_dim[i] = 0;
+    v->num_parts[i] = 0;
+    v->first[i] = 0;
+    v->split_val[i] = v->m[i] = v->cmx[i] = v->cmy[i] = v->cmz[i] = v->size[i] = 0.0;
+    v->left[i] = v->right[i] = 0;
+  }
+  for (int64_t i = 0; i < n; ++i) v->perm[i] = oid[i];
+  if (layout == KD_LAYOUT_PADDED) {
+    if (!emit_padded(t, 0, 0, need, v))
+      return fail(KD_ERR_CAPACITY, "padded layout overflows its %lld-node allocation",
+                  (long long)need);
+  } else {
+    emit_compact(t, 0, 0, v);
+  }
+  v->n_nodes = need;
+  return KD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// initial conditions (host)
+// ---------------------------------------------------------------------------
+
+static uint64_t sm64(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static double unif(uint64_t &s) { return (double)(sm64(s) >> 11) * 0x1.0p-53; }
+
+int kd_ic_ring(uint64_t seed, int64_t n, double *x, double *y, double *z, double *vx,
+               double *vy, double *vz, double *m) {
+  if (n < 1 || !x || !y || !z || !vx || !vy || !vz || !m)
+    return fail(KD_ERR_ARG, "bad argument to kd_ic_ring");
+  uint64_t s = seed;
+  const double two_pi = 6.283185307179586;
+  x[0] = y[0] = z[0] = vx[0] = vy[0] = vz[0] = 0.0;
+  m[0] = 1.0;
+  for (int64_t i = 1; i < n; ++i) {
+    double r = 1.0 + unif(s);
+    double phi = two_pi * unif(s);
+    double u1 = unif(s), u2 = unif(s);
+    double g = std::sqrt(-2.0 * std::log(1.0 - u1)) * std::cos(two_pi * u2);
+    double v = std::sqrt(1.0 / r);
+    double c = std::cos(phi), sn = std::sin(phi);
+    x[i] = r * c;
+    y[i] = r * sn;
+    z[i] = 1e-3 * g;
+    vx[i] = -v * sn;
+    vy[i] = v * c;
+    vz[i] = 0.0;
+    m[i] = 1e-14;
+  }
+  return KD_OK;
+}
+
+int kd_ic_circular_orbits(uint64_t seed, int64_t n, double *x, double *y, double *z,
+                          double *vx, double *vy, double *vz, double *m) {
+  if (n < 1 || !x || !y || !z || !vx || !vy || !vz || !m)
+    return fail(KD_ERR_ARG, "bad argument to kd_ic_circular_orbits");
+  uint64_t s = seed;
+  x[0] = y[0] = z[0] = vx[0] = vy[0] = vz[0] = 0.0;
+  m[0] = 1.0;
+  for (int64_t i = 0; i < n; ++i) {
+    double d = 0.1 + ((double)i * 5.0 / (double)n);
+    double v = std::sqrt(1.0 / d);
+    double theta = unif(s) * 6.28;
+    x[i + 1] = d * std::cos(theta);
+    y[i + 1] = d * std::sin(theta);
+    z[i + 1] = 0.0;
+    vx[i + 1] = -v * std::sin(theta);
+    vy[i + 1] = v * std::cos(theta);
+    vz[i + 1] = 0.0;
+    m[i + 1] = 1e-14;
+  }
+  return KD_OK;
+}
+
+int kd_ic_two_bodies(double *x, double *y, double *z, double *vx, double *vy, double *vz,
+                     double *m) {
+  if (!x || !y || !z || !vx || !vy || !vz || !m)
+    return fail(KD_ERR_ARG, "bad argument to kd_ic_two_bodies");
+  x[0] = y[0] = z[0] = vx[0] = vy[0] = vz[0] = 0.0;
+  m[0] = 1.0;
+  x[1] = 1.0;
+  y[1] = z[1] = 0.0;
+  vx[1] = 0.0;
+  vy[1] = 1.0;
+  vz[1] = 0.0;
+  m[1] = 1e-20;
+  return KD_OK;
+}
+
+}  // extern "C"

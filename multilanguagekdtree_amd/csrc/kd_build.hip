@@ -1,0 +1,850 @@
+// kd_build.hip -- gfx950 tree build: level-synchronous median partition.
+//
+// The tree shape is a pure function of N (every split is at floor(len/2),
+// Sequential/RustVersion/src/array_kd_tree.rs:100), so segment bounds are
+// known up front and the tree is kept in heap order.  Each step:
+//   1. three lists of body ids sorted by (x), (y), (z) (device radix sort);
+//   2. device-wide levels while segments exceed SUB_MAX bodies: per segment,
+//      bbox from the ends of the three sorted lists, split dimension with the
+//      reference's strict '>' (:91-96), split value = key at mid (:125); the
+//      split-dim list is already partitioned at mid, the other two are
+//      stably partitioned by a left/right flag (segmented scan + scatter);
+//   3. one workgroup per remaining segment finishes its subtree in LDS,
+//      emits leaves in canonical order (ascending original index), node
+//      moments bottom-up, and writes the body state in tree order;
+//   4. one workgroup reduces the moments of the device-wide levels.
+// Ties on a split coordinate are resolved by original index, lazily: only a
+// run of equal keys that straddles the median is reordered.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "kd_internal.hpp"
+
+namespace kd {
+
+__device__ __forceinline__ uint64_t ordkey(double v) {
+  uint64_t b = (uint64_t)__double_as_longlong(v);
+  if ((b << 1) == 0) b = 0;  // -0.0 and +0.0 compare equal
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// ---------------------------------------------------------------------------
+// block scans
+// ---------------------------------------------------------------------------
+
+// Exclusive sum over the block; total receives the block sum.  sh needs
+// NT/64 entries.
+template <int NT, typename T>
+__device__ __forceinline__ T block_excl_sum(T v, T *sh, T &total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    T y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    T t = lane < NT / 64 ? sh[lane] : T(0);
+#pragma unroll
+    for (int o = 1; o < NT / 64; o <<= 1) {
+      T y = __shfl_up(t, o, 64);
+      if (lane >= o) t += y;
+    }
+    if (lane < NT / 64) sh[lane] = t;
+  }
+  __syncthreads();
+  T before = w ? sh[w - 1] : T(0);
+  total = sh[NT / 64 - 1];
+  __syncthreads();
+  return before + x - v;
+}
+
+// Segmented inclusive-scan element: value and "segment starts here" flag.
+struct SegV {
+  uint64_t v;
+  uint32_t h;
+};
+__device__ __forceinline__ SegV seg_combine(SegV a, SegV b) {
+  SegV r;
+  r.v = b.h ? b.v : a.v + b.v;
+  r.h = a.h | b.h;
+  return r;
+}
+
+// Exclusive segmented carry for each thread: combination of all earlier
+// threads' aggregates.  shv/shh need NT/64 entries.
+template <int NT>
+__device__ __forceinline__ SegV block_seg_carry(SegV agg, uint64_t *shv,
+                                                uint32_t *shh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  SegV x = agg;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    SegV y;
+    y.v = __shfl_up(x.v, o, 64);
+    y.h = __shfl_up(x.h, o, 64);
+    if (lane >= o) x = seg_combine(y, x);
+  }
+  if (lane == 63) {
+    shv[w] = x.v;
+    shh[w] = x.h;
+  }
+  __syncthreads();
+  if (w == 0) {
+    SegV t;
+    t.v = lane < NT / 64 ? shv[lane] : 0;
+    t.h = lane < NT / 64 ? shh[lane] : 0;
+#pragma unroll
+    for (int o = 1; o < NT / 64; o <<= 1) {
+      SegV y;
+      y.v = __shfl_up(t.v, o, 64);
+      y.h = __shfl_up(t.h, o, 64);
+      if (lane >= o) t = seg_combine(y, t);
+    }
+    if (lane < NT / 64) {
+      shv[lane] = t.v;
+      shh[lane] = t.h;
+    }
+  }
+  __syncthreads();
+  // exclusive within the wave
+  SegV prev;
+  prev.v = __shfl_up(x.v, 1, 64);
+  prev.h = __shfl_up(x.h, 1, 64);
+  if (lane == 0) {
+    prev.v = 0;
+    prev.h = 0;
+  }
+  SegV carry;
+  carry.v = w ? shv[w - 1] : 0;
+  carry.h = w ? shh[w - 1] : 0;
+  SegV r = lane == 0 ? carry : seg_combine(carry, prev);
+  __syncthreads();
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// shape
+// ---------------------------------------------------------------------------
+
+// Segment of every heap slot: descend from the root splitting at floor(c/2);
+// slots below a leaf get count 0.
+__global__ void k_shape(uint32_t *seg_start, uint32_t *seg_count, int64_t slots,
+                        uint32_t n, int leaf) {
+  int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= slots) return;
+  uint32_t hp = (uint32_t)h + 1;
+  int lev = 31 - __clz(hp);
+  uint32_t k = hp - (1u << lev);
+  uint32_t s = 0, c = n;
+  bool ok = true;
+  for (int b = lev - 1; b >= 0; --b) {
+    if (c <= (uint32_t)leaf) {
+      ok = false;
+      break;
+    }
+    uint32_t half = c >> 1;
+    if ((k >> b) & 1u) {
+      s += half;
+      c -= half;
+    } else {
+      c = half;
+    }
+  }
+  seg_start[h] = ok ? s : 0;
+  seg_count[h] = ok ? c : 0;
+}
+
+// ---------------------------------------------------------------------------
+// sort keys
+// ---------------------------------------------------------------------------
+
+__global__ void k_make_keys(const double *__restrict__ p, uint64_t *__restrict__ keys,
+                            uint32_t *__restrict__ ids, uint32_t n) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  keys[j] = ordkey(p[j]);
+  ids[j] = j;
+}
+
+// ---------------------------------------------------------------------------
+// device-wide levels
+// ---------------------------------------------------------------------------
+
+struct Lists {
+  uint32_t *l[3];
+};
+struct Coords {
+  const double *c[3];
+};
+
+__global__ void k_level_split(Lists L, Coords P, TreeDev t, uint32_t level,
+                              uint32_t *tieflag) {
+  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t nseg = 1u << level;
+  if (k >= nseg) return;
+  uint32_t h = nseg - 1 + k;
+  uint32_t s = t.seg_start[h], c = t.seg_count[h];
+  double ext[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double lo = P.c[d][L.l[d][s]];
+    double hi = P.c[d][L.l[d][s + c - 1]];
+    ext[d] = hi - lo;
+  }
+  int sd = 0;
+  if (ext[1] > ext[sd]) sd = 1;
+  if (ext[2] > ext[sd]) sd = 2;
+  uint32_t mid = s + c / 2;
+  double kv = P.c[sd][L.l[sd][mid]];
+  double kp = P.c[sd][L.l[sd][mid - 1]];
+  t.split_dim[h] = (uint8_t)sd;
+  t.split_val[h] = kv;
+  t.size[h] = ext[sd];
+  tieflag[k] = (kp == kv) ? 1u : 0u;
+}
+
+// A run of equal keys straddles the median: the run's members that go left
+// are those with the smallest original indices.  One 1024-thread block per
+// flagged segment: radix-select the threshold index, then stably partition
+// the run (scratch = the same list's output buffer, overwritten later).
+__global__ __launch_bounds__(1024) void k_level_tiefix(Lists L, Lists scratch,
+                                                       Coords P, TreeDev t,
+                                                       const uint32_t *oid,
+                                                       uint32_t level,
+                                                       const uint32_t *tieflag) {
+  constexpr int NT = 1024;
+  uint32_t k = blockIdx.x;
+  if (!tieflag[k]) return;
+  uint32_t h = (1u << level) - 1 + k;
+  uint32_t s = t.seg_start[h], c = t.seg_count[h];
+  int sd = t.split_dim[h];
+  uint32_t *ls = L.l[sd];
+  uint32_t *tmp = scratch.l[sd];
+  const double *key = P.c[sd];
+  uint32_t mid = s + c / 2;
+  double kv = key[ls[mid]];
+  __shared__ uint32_t r0s, r1s, need_s, prefix_s, mask_s;
+  __shared__ uint32_t hist[256];
+  __shared__ uint32_t scan_sh[NT / 64];
+  if (threadIdx.x == 0) {
+    // keys are sorted non-decreasing inside [s, s+c)
+    uint32_t lo = s, hi = mid;  // first position with key == kv
+    while (lo < hi) {
+      uint32_t m = lo + (hi - lo) / 2;
+      if (key[ls[m]] < kv) lo = m + 1; else hi = m;
+    }
+    uint32_t r0 = lo;
+    lo = mid + 1;
+    hi = s + c;  // first position with key > kv
+    while (lo < hi) {
+      uint32_t m = lo + (hi - lo) / 2;
+      if (key[ls[m]] > kv) hi = m; else lo = m + 1;
+    }
+    r0s = r0;
+    r1s = lo;
+    need_s = mid - r0;
+    prefix_s = 0;
+    mask_s = 0;
+  }
+  __syncthreads();
+  const uint32_t r0 = r0s, r1 = r1s;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    for (int i = threadIdx.x; i < 256; i += NT) hist[i] = 0;
+    __syncthreads();
+    const uint32_t pre = prefix_s, msk = mask_s;
+    for (uint32_t p = r0 + threadIdx.x; p < r1; p += NT) {
+      uint32_t o = oid[ls[p]];
+      if ((o & msk) == pre) atomicAdd(&hist[(o >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t rem = need_s, cum = 0, b = 0;
+      for (; b < 255; ++b) {
+        if (cum + hist[b] > rem) break;
+        cum += hist[b];
+      }
+      need_s = rem - cum;
+      prefix_s = pre | (b << shift);
+      mask_s = msk | (255u << shift);
+    }
+    __syncthreads();
+  }
+  const uint32_t thr = prefix_s;  // original index of the first body that goes right
+  uint32_t nleft = mid - r0;
+  uint32_t off_l = 0, off_r = nleft;
+  for (uint32_t base = r0; base < r1; base += NT) {
+    uint32_t p = base + threadIdx.x;
+    uint32_t id = p < r1 ? ls[p] : 0;
+    uint32_t f = (p < r1 && oid[id] < thr) ? 1u : 0u;
+    uint32_t tot;
+    uint32_t ex = block_excl_sum<NT>(f, scan_sh, tot);
+    if (p < r1) {
+      uint32_t idx = p - base;
+      uint32_t dst = f ? off_l + ex : off_r + (idx - ex);
+      tmp[r0 + dst] = id;
+    }
+    uint32_t cnt = min((uint32_t)NT, r1 - base);
+    off_l += tot;
+    off_r += cnt - tot;
+  }
+  __syncthreads();
+  for (uint32_t p = r0 + threadIdx.x; p < r1; p += NT) ls[p] = tmp[p];
+}
+
+__global__ __launch_bounds__(LVL_THREADS) void k_level_flags(Lists L, TreeDev t,
+                                                             uint32_t level,
+                                                             uint32_t T,
+                                                             uint8_t *flags) {
+  uint32_t k = blockIdx.x / T, q = blockIdx.x % T;
+  uint32_t h = (1u << level) - 1 + k;
+  uint32_t s = t.seg_start[h], c = t.seg_count[h];
+  int sd = t.split_dim[h];
+  uint32_t mid = s + c / 2, e = s + c;
+  const uint32_t *ls = L.l[sd];
+  uint32_t base = s + q * LVL_TILE;
+  for (int i = 0; i < LVL_ITEMS; ++i) {
+    uint32_t j = base + i * LVL_THREADS + threadIdx.x;
+    if (j < e) flags[ls[j]] = j < mid ? 1 : 0;
+  }
+}
+
+__global__ __launch_bounds__(LVL_THREADS) void k_level_count(Lists L, TreeDev t,
+                                                             uint32_t level,
+                                                             uint32_t T,
+                                                             const uint8_t *flags,
+                                                             uint32_t *cnt) {
+  __shared__ uint32_t sh[LVL_THREADS / 64];
+  uint32_t nseg = 1u << level;
+  uint32_t k = blockIdx.x / T, q = blockIdx.x % T;
+  uint32_t h = nseg - 1 + k;
+  uint32_t s = t.seg_start[h], c = t.seg_count[h];
+  int sd = t.split_dim[h];
+  uint32_t e = s + c;
+  uint32_t base = s + q * LVL_TILE;
+  for (int d = 0; d < 3; ++d) {
+    uint32_t local = 0;
+    if (d != sd) {
+      const uint32_t *ld = L.l[d];
+      for (int i = 0; i < LVL_ITEMS; ++i) {
+        uint32_t j = base + i * LVL_THREADS + threadIdx.x;
+        if (j < e) local += flags[ld[j]];
+      }
+    }
+    uint32_t tot;
+    block_excl_sum<LVL_THREADS>(local, sh, tot);
+    if (threadIdx.x == 0) cnt[((size_t)d * nseg + k) * T + q] = tot;
+  }
+}
+
+// Exclusive scan over the T tile counts of one (list, segment).
+__global__ __launch_bounds__(256) void k_level_scan(uint32_t *cnt, uint32_t T) {
+  __shared__ uint32_t sh[4];
+  uint32_t *row = cnt + (size_t)blockIdx.x * T;
+  uint32_t carry = 0;
+  for (uint32_t b = 0; b < T; b += 256) {
+    uint32_t i = b + threadIdx.x;
+    uint32_t v = i < T ? row[i] : 0;
+    uint32_t tot;
+    uint32_t ex = block_excl_sum<256>(v, sh, tot);
+    if (i < T) row[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(Lists L, Lists Lout,
+                                                               TreeDev t,
+                                                               uint32_t level,
+                                                               uint32_t T,
+                                                               const uint8_t *flags,
+                                                               const uint32_t *cnt) {
+  __shared__ uint32_t sh[LVL_THREADS / 64];
+  uint32_t nseg = 1u << level;
+  uint32_t k = blockIdx.x / T, q = blockIdx.x % T;
+  uint32_t h = nseg - 1 + k;
+  uint32_t s = t.seg_start[h], c = t.seg_count[h];
+  int sd = t.split_dim[h];
+  uint32_t mid = s + c / 2, e = s + c;
+  uint32_t base = s + q * LVL_TILE + threadIdx.x * LVL_ITEMS;
+  for (int d = 0; d < 3; ++d) {
+    const uint32_t *ld = L.l[d];
+    uint32_t *od = Lout.l[d];
+    if (d == sd) {
+      for (int i = 0; i < LVL_ITEMS; ++i) {
+        uint32_t j = s + q * LVL_TILE + i * LVL_THREADS + threadIdx.x;
+        if (j < e) od[j] = ld[j];
+      }
+      continue;
+    }
+    uint32_t id[LVL_ITEMS];
+    uint32_t f[LVL_ITEMS];
+    uint32_t local = 0;
+#pragma unroll
+    for (int i = 0; i < LVL_ITEMS; ++i) {
+      uint32_t j = base + i;
+      id[i] = j < e ? ld[j] : 0;
+      f[i] = j < e ? flags[id[i]] : 0;
+      local += f[i];
+    }
+    uint32_t tot;
+    uint32_t ex = block_excl_sum<LVL_THREADS>(local, sh, tot);
+    uint32_t before = cnt[((size_t)d * nseg + k) * T + q] + ex;  // lefts before j in segment
+#pragma unroll
+    for (int i = 0; i < LVL_ITEMS; ++i) {
+      uint32_t j = base + i;
+      if (j < e) {
+        uint32_t dst = f[i] ? s + before : mid + (j - s - before);
+        od[dst] = id[i];
+      }
+      before += f[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// subtree builder: one workgroup per segment of <= SUB_MAX bodies
+// ---------------------------------------------------------------------------
+
+struct SubArgs {
+  Lists L;          // sorted lists (device-wide levels done)
+  Coords P;         // positions, working order
+  const double *m;  // masses, working order
+  StateDev in;      // state, working order
+  StateDev out;     // state, tree order
+  TreeDev t;
+  uint32_t *loc;    // scratch: working id -> local index
+  uint32_t *perm;   // tree position -> working id
+  double4 *pos64;   // tree order (x, y, z, m)
+  float4 *pos32;
+  uint32_t level;
+  int leaf;
+};
+
+__device__ void write_leaf_rec(const TreeDev &t, uint32_t h, uint32_t start,
+                               uint32_t count) {
+  NodeRec64 r;
+  r.cx = r.cy = r.cz = r.m = r.size2 = 0.0;
+  r.start = start;
+  r.count = count;
+  r.pad0 = r.pad1 = 0.0;
+  t.rec64[h] = r;
+  NodeRec32 q;
+  q.cx = q.cy = q.cz = q.m = q.size2 = 0.f;
+  q.start = start;
+  q.count = count;
+  q.pad = 0.f;
+  t.rec32[h] = q;
+  NodeRecD dd;
+  dd.cx = dd.cy = dd.cz = 0.0;
+  dd.m = dd.size2 = 0.f;
+  dd.start = start;
+  dd.count = count;
+  dd.pad0 = dd.pad1 = 0.0;
+  t.recd[h] = dd;
+}
+
+__device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
+                                   uint32_t count) {
+#pragma clang fp contract(off)
+  double m = t.mm[h];
+  double cx = t.mx[h] / m, cy = t.my[h] / m, cz = t.mz[h] / m;
+  double sz = t.size[h];
+  double s2 = sz * sz;
+  NodeRec64 r;
+  r.cx = cx;
+  r.cy = cy;
+  r.cz = cz;
+  r.m = m;
+  r.size2 = s2;
+  r.start = start;
+  r.count = count;
+  r.pad0 = r.pad1 = 0.0;
+  t.rec64[h] = r;
+  NodeRec32 q;
+  q.cx = (float)cx;
+  q.cy = (float)cy;
+  q.cz = (float)cz;
+  q.m = (float)m;
+  q.size2 = (float)s2;
+  q.start = start;
+  q.count = count;
+  q.pad = 0.f;
+  t.rec32[h] = q;
+  NodeRecD dd;
+  dd.cx = cx;
+  dd.cy = cy;
+  dd.cz = cz;
+  dd.m = (float)m;
+  dd.size2 = (float)s2;
+  dd.start = start;
+  dd.count = count;
+  dd.pad0 = dd.pad1 = 0.0;
+  t.recd[h] = dd;
+}
+
+__global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
+#pragma clang fp contract(off)
+  constexpr int NT = SUB_THREADS, IT = SUB_ITEMS;
+  __shared__ uint32_t ids[SUB_MAX];
+  __shared__ uint16_t lst[2][3][SUB_MAX];
+  __shared__ uint8_t flag[SUB_MAX];
+  __shared__ uint8_t sdarr[SUB_MAX];
+  __shared__ uint64_t shv[NT / 64];
+  __shared__ uint32_t shh[NT / 64];
+
+  const uint32_t k = blockIdx.x;
+  const uint32_t h0 = (1u << a.level) - 1 + k;
+  const uint32_t s0 = a.t.seg_start[h0], n0 = a.t.seg_count[h0];
+  if (n0 == 0) return;
+  const int leaf = a.leaf;
+  const int tid = threadIdx.x;
+
+  for (uint32_t i = tid; i < n0; i += NT) {
+    uint32_t id = a.L.l[0][s0 + i];
+    ids[i] = id;
+    lst[0][0][i] = (uint16_t)i;
+    a.loc[id] = i;
+  }
+  __threadfence_block();
+  __syncthreads();
+  for (uint32_t i = tid; i < n0; i += NT) {
+    lst[0][1][i] = (uint16_t)a.loc[a.L.l[1][s0 + i]];
+    lst[0][2][i] = (uint16_t)a.loc[a.L.l[2][s0 + i]];
+  }
+  __syncthreads();
+
+  uint32_t st[IT], ct[IT], cc[IT];
+  bool alive[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    uint32_t j = tid * IT + it;
+    alive[it] = j < n0;
+    st[it] = 0;
+    ct[it] = n0;
+    cc[it] = 0;
+  }
+  int cur = 0;
+  int lam_end = 0;
+  for (int lam = 0;; ++lam) {
+    const uint32_t hbase = (1u << (a.level + lam)) - 1 + (k << lam);
+    // segment owners: leaf emission or split decision
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      uint32_t j = tid * IT + it;
+      if (!alive[it] || j != st[it]) continue;
+      uint32_t h = hbase + cc[it];
+      uint32_t sgs = st[it], sgc = ct[it];
+      if (sgc <= (uint32_t)leaf) {
+        // canonical member order: ascending original index (insertion sort
+        // of the leaf's slice of list 0, at most MAX_LEAF entries)
+        uint16_t *l0 = &lst[cur][0][sgs];
+        for (uint32_t x = 1; x < sgc; ++x) {
+          uint16_t v = l0[x];
+          uint32_t ov = a.in.oid[ids[v]];
+          int y = (int)x - 1;
+          while (y >= 0 && a.in.oid[ids[l0[y]]] > ov) {
+            l0[y + 1] = l0[y];
+            --y;
+          }
+          l0[y + 1] = v;
+        }
+        double mm = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+        for (uint32_t x = 0; x < sgc; ++x) {
+          uint32_t id = ids[l0[x]];
+          double mi = a.m[id];
+          mm += mi;
+          mx += mi * a.P.c[0][id];
+          my += mi * a.P.c[1][id];
+          mz += mi * a.P.c[2][id];
+        }
+        a.t.mm[h] = mm;
+        a.t.mx[h] = mx;
+        a.t.my[h] = my;
+        a.t.mz[h] = mz;
+        write_leaf_rec(a.t, h, s0 + sgs, sgc);
+        sdarr[sgs] = 0xFF;
+      } else {
+        double ext[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double lo = a.P.c[d][ids[lst[cur][d][sgs]]];
+          double hi = a.P.c[d][ids[lst[cur][d][sgs + sgc - 1]]];
+          ext[d] = hi - lo;
+        }
+        int sd = 0;
+        if (ext[1] > ext[sd]) sd = 1;
+        if (ext[2] > ext[sd]) sd = 2;
+        uint32_t mid = sgs + sgc / 2;
+        uint16_t *ls = lst[cur][sd];
+        const double *key = a.P.c[sd];
+        double kv = key[ids[ls[mid]]];
+        if (key[ids[ls[mid - 1]]] == kv) {
+          // equal keys straddle the median: order the run by original index
+          uint32_t r0 = mid, r1 = mid + 1;
+          while (r0 > sgs && key[ids[ls[r0 - 1]]] == kv) --r0;
+          while (r1 < sgs + sgc && key[ids[ls[r1]]] == kv) ++r1;
+          for (uint32_t x = r0 + 1; x < r1; ++x) {
+            uint16_t v = ls[x];
+            uint32_t ov = a.in.oid[ids[v]];
+            int y = (int)x - 1;
+            while (y >= (int)r0 && a.in.oid[ids[ls[y]]] > ov) {
+              ls[y + 1] = ls[y];
+              --y;
+            }
+            ls[y + 1] = v;
+          }
+        }
+        a.t.split_dim[h] = (uint8_t)sd;
+        a.t.split_val[h] = kv;
+        a.t.size[h] = ext[sd];
+        sdarr[sgs] = (uint8_t)sd;
+      }
+    }
+    __syncthreads();
+    int any = 0;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      if (alive[it] && sdarr[st[it]] == 0xFF) alive[it] = false;
+      any |= alive[it] ? 1 : 0;
+    }
+    lam_end = lam;
+    if (!__syncthreads_or(any)) break;
+
+    // flags from the split-dim list
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      if (!alive[it]) continue;
+      uint32_t j = tid * IT + it;
+      int sd = sdarr[st[it]];
+      uint32_t mid = st[it] + ct[it] / 2;
+      flag[lst[cur][sd][j]] = j < mid ? 1 : 0;
+    }
+    __syncthreads();
+
+    // segmented exclusive scan of the left flags, three lists packed
+    uint64_t v[IT];
+    uint32_t hd[IT];
+    SegV agg;
+    agg.v = 0;
+    agg.h = 0;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      uint32_t j = tid * IT + it;
+      v[it] = 0;
+      hd[it] = 1;
+      if (alive[it]) {
+        int sd = sdarr[st[it]];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          if (d != sd) v[it] += (uint64_t)flag[lst[cur][d][j]] << (16 * d);
+        hd[it] = (j == st[it]) ? 1u : 0u;
+      }
+      SegV e;
+      e.v = v[it];
+      e.h = hd[it];
+      agg = seg_combine(agg, e);
+    }
+    SegV carry = block_seg_carry<NT>(agg, shv, shh);
+    uint64_t run = carry.v;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      uint32_t j = tid * IT + it;
+      if (hd[it]) run = 0;
+      uint64_t ex = run;  // lefts before j in its segment, per list
+      run += v[it];
+      if (j >= n0) continue;
+      int sd = alive[it] ? sdarr[st[it]] : -1;
+      uint32_t mid = st[it] + ct[it] / 2;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        uint16_t x = lst[cur][d][j];
+        uint32_t dst = j;
+        if (alive[it] && d != sd) {
+          uint32_t e = (uint32_t)((ex >> (16 * d)) & 0xFFFFu);
+          dst = flag[x] ? st[it] + e : mid + (j - st[it] - e);
+        }
+        lst[cur ^ 1][d][dst] = x;
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      if (!alive[it]) continue;
+      uint32_t j = tid * IT + it;
+      uint32_t mid = st[it] + ct[it] / 2;
+      if (j < mid) {
+        ct[it] = ct[it] / 2;
+        cc[it] = 2 * cc[it];
+      } else {
+        ct[it] = ct[it] - ct[it] / 2;
+        st[it] = mid;
+        cc[it] = 2 * cc[it] + 1;
+      }
+    }
+  }
+
+  // tree-order body state; list 0 now holds every leaf's members in order
+  for (uint32_t i = tid; i < n0; i += NT) {
+    uint32_t w = ids[lst[cur][0][i]];
+    uint32_t tpos = s0 + i;
+    a.perm[tpos] = w;
+    double x = a.in.x[w], y = a.in.y[w], z = a.in.z[w], m = a.in.m[w];
+    a.out.x[tpos] = x;
+    a.out.y[tpos] = y;
+    a.out.z[tpos] = z;
+    a.out.vx[tpos] = a.in.vx[w];
+    a.out.vy[tpos] = a.in.vy[w];
+    a.out.vz[tpos] = a.in.vz[w];
+    a.out.m[tpos] = m;
+    a.out.oid[tpos] = a.in.oid[w];
+    a.pos64[tpos] = make_double4(x, y, z, m);
+    a.pos32[tpos] = make_float4((float)x, (float)y, (float)z, (float)m);
+  }
+
+  // moments bottom-up over this subtree's internal nodes
+  __threadfence_block();
+  __syncthreads();
+  for (int lam = lam_end - 1; lam >= 0; --lam) {
+    const uint32_t hbase = (1u << (a.level + lam)) - 1 + (k << lam);
+    for (uint32_t c = tid; c < (1u << lam); c += NT) {
+      uint32_t s = 0, cnt = n0;
+      bool ok = true;
+      for (int b = lam - 1; b >= 0; --b) {
+        if (cnt <= (uint32_t)leaf) {
+          ok = false;
+          break;
+        }
+        uint32_t half = cnt >> 1;
+        if ((c >> b) & 1u) {
+          s += half;
+          cnt -= half;
+        } else {
+          cnt = half;
+        }
+      }
+      if (!ok || cnt <= (uint32_t)leaf) continue;
+      uint32_t h = hbase + c;
+      uint32_t l = 2 * h + 1, r = 2 * h + 2;
+      a.t.mm[h] = a.t.mm[l] + a.t.mm[r];
+      a.t.mx[h] = a.t.mx[l] + a.t.mx[r];
+      a.t.my[h] = a.t.my[l] + a.t.my[r];
+      a.t.mz[h] = a.t.mz[l] + a.t.mz[r];
+      write_internal_rec(a.t, h, s0 + s, cnt);
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// Moments and records of the device-wide levels [0, level), bottom-up.
+__global__ __launch_bounds__(1024) void k_moments_top(TreeDev t, uint32_t level) {
+#pragma clang fp contract(off)
+  for (int lev = (int)level - 1; lev >= 0; --lev) {
+    uint32_t nseg = 1u << lev;
+    for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x) {
+      uint32_t h = nseg - 1 + k;
+      uint32_t l = 2 * h + 1, r = 2 * h + 2;
+      t.mm[h] = t.mm[l] + t.mm[r];
+      t.mx[h] = t.mx[l] + t.mx[r];
+      t.my[h] = t.my[l] + t.my[r];
+      t.mz[h] = t.mz[l] + t.mz[r];
+      write_internal_rec(t, h, t.seg_start[h], t.seg_count[h]);
+    }
+    __threadfence_block();
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+
+Shape make_shape(int64_t n, int leaf) {
+  Shape s;
+  s.n = n;
+  s.leaf = leaf;
+  int l = 0;
+  int64_t mx = n;
+  while (mx > leaf) {
+    mx = (mx + 1) / 2;
+    ++l;
+  }
+  s.levels = l + 1;
+  int ls = 0;
+  mx = n;
+  while (mx > SUB_MAX) {
+    mx = (mx + 1) / 2;
+    ++ls;
+  }
+  s.sub_level = ls;
+  s.slots = ((int64_t)1 << s.levels) - 1;
+  return s;
+}
+
+
+size_t sort_temp_bytes(uint32_t n) {
+  size_t bytes = 0;
+  (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                            (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 64);
+  return bytes;
+}
+
+hipError_t launch_shape(const TreeDev &t, const Shape &sh, hipStream_t s) {
+  int64_t blocks = (sh.slots + 255) / 256;
+  k_shape<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(t.seg_start, t.seg_count, sh.slots,
+                                                      (uint32_t)sh.n, sh.leaf);
+  return hipGetLastError();
+}
+
+hipError_t launch_build(const Shape &sh, const StateDev &in, const StateDev &out,
+                        const TreeDev &t, BuildBufs &b, double4 *pos64,
+                        float4 *pos32, hipStream_t s) {
+  const uint32_t n = (uint32_t)sh.n;
+  const double *coord[3] = {in.x, in.y, in.z};
+  hipError_t e;
+  if (sort_temp_bytes(n) > b.sort_tmp_bytes) return hipErrorInvalidValue;
+  for (int d = 0; d < 3; ++d) {
+    k_make_keys<<<(n + 255) / 256, 256, 0, s>>>(coord[d], b.keys_in, b.ids_in, n);
+    size_t bytes = b.sort_tmp_bytes;
+    e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, b.keys_in, b.keys_out, b.ids_in, b.L[d],
+                                  n, 0, 64, s);
+    if (e != hipSuccess) return e;
+  }
+  Lists L{{b.L[0], b.L[1], b.L[2]}};
+  Lists Lb{{b.Lb[0], b.Lb[1], b.Lb[2]}};
+  Coords P{{in.x, in.y, in.z}};
+  for (int lev = 0; lev < sh.sub_level; ++lev) {
+    uint32_t nseg = 1u << lev;
+    uint32_t maxseg = (uint32_t)((sh.n + nseg - 1) / nseg);
+    uint32_t T = (maxseg + LVL_TILE - 1) / LVL_TILE;
+    k_level_split<<<(nseg + 255) / 256, 256, 0, s>>>(L, P, t, lev, b.tieflag);
+    k_level_tiefix<<<nseg, 1024, 0, s>>>(L, Lb, P, t, in.oid, lev, b.tieflag);
+    k_level_flags<<<nseg * T, LVL_THREADS, 0, s>>>(L, t, lev, T, b.flags);
+    k_level_count<<<nseg * T, LVL_THREADS, 0, s>>>(L, t, lev, T, b.flags, b.cnt);
+    k_level_scan<<<3 * nseg, 256, 0, s>>>(b.cnt, T);
+    k_level_scatter<<<nseg * T, LVL_THREADS, 0, s>>>(L, Lb, t, lev, T, b.flags, b.cnt);
+    Lists tmp = L;
+    L = Lb;
+    Lb = tmp;
+  }
+  SubArgs a;
+  a.L = L;
+  a.P = P;
+  a.m = in.m;
+  a.in = in;
+  a.out = out;
+  a.t = t;
+  a.loc = b.loc;
+  a.perm = b.perm;
+  a.pos64 = pos64;
+  a.pos32 = pos32;
+  a.level = sh.sub_level;
+  a.leaf = sh.leaf;
+  k_subtree<<<1u << sh.sub_level, SUB_THREADS, 0, s>>>(a);
+  if (sh.sub_level > 0) k_moments_top<<<1, 1024, 0, s>>>(t, sh.sub_level);
+  return hipGetLastError();
+}
+
+}  // namespace kd

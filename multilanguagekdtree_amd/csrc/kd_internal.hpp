@@ -1,0 +1,113 @@
+// kd_internal.hpp -- shared layouts and host-side launch declarations of the
+// gfx950 kD-tree Barnes-Hut step.  Device code lives in kd_build.hip,
+// kd_traverse.hip and kd_misc.hip; the C ABI in kd_api.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kd {
+
+// Largest segment the one-workgroup subtree builder holds in LDS, and its
+// launch shape (SUB_THREADS threads x SUB_ITEMS consecutive positions).
+constexpr int SUB_THREADS = 512;
+constexpr int SUB_ITEMS = 8;
+constexpr int SUB_MAX = SUB_THREADS * SUB_ITEMS;  // 4096
+
+// Positions per block in the device-wide level kernels.
+constexpr int LVL_THREADS = 256;
+constexpr int LVL_ITEMS = 8;
+constexpr int LVL_TILE = LVL_THREADS * LVL_ITEMS;  // 2048
+
+constexpr int MAX_LEAF = 32;
+
+// Node record read by the FP64 traversal: 64 bytes, one scalar load.
+// Internal nodes: centre of mass, mass, size*size, and the particle range of
+// the node.  Leaves: the particle range only (count <= leaf size).
+struct alignas(64) NodeRec64 {
+  double cx, cy, cz, m;
+  double size2;
+  uint32_t start, count;
+  double pad0, pad1;
+};
+
+// Node record read by the mixed-precision traversal: 32 bytes.
+struct alignas(32) NodeRec32 {
+  float cx, cy, cz, m;
+  float size2;
+  uint32_t start, count;
+  float pad;
+};
+
+// Node record for the mixed traversal with FP64 coordinate differences.
+struct alignas(64) NodeRecD {
+  double cx, cy, cz;
+  float m, size2;
+  uint32_t start, count;
+  double pad0, pad1;
+};
+
+// Device-resident tree of one system in heap order: node h has children
+// 2h+1 and 2h+2; level l holds heap slots [2^l - 1, 2^(l+1) - 1).
+struct TreeDev {
+  uint32_t *seg_start;  // first tree position of the node's bodies
+  uint32_t *seg_count;  // number of bodies (0 for unused heap slots)
+  uint8_t *split_dim;
+  double *split_val;
+  double *size;
+  double *mm, *mx, *my, *mz;  // mass and mass-weighted position sums
+  NodeRec64 *rec64;
+  NodeRec32 *rec32;
+  NodeRecD *recd;
+};
+
+// Body state in "working order" (the previous step's tree order).
+struct StateDev {
+  double *x, *y, *z, *vx, *vy, *vz, *m;
+  uint32_t *oid;  // original index of the body held at this position
+};
+
+struct Shape {
+  int64_t n = 0;
+  int leaf = 7;
+  int levels = 0;       // deepest level + 1
+  int sub_level = 0;    // level whose segments the subtree builder takes
+  int64_t slots = 0;    // heap slots (2^levels - 1)
+};
+
+struct BuildBufs {
+  uint32_t *L[3], *Lb[3];  // sorted id lists and their partition buffers
+  uint64_t *keys_in, *keys_out;
+  uint32_t *ids_in;
+  void *sort_tmp;
+  size_t sort_tmp_bytes;
+  uint8_t *flags;    // left/right flag per working id
+  uint32_t *loc;     // working id -> local index inside a subtree block
+  uint32_t *cnt;     // per-tile left counts of the device-wide levels
+  uint32_t *tieflag; // per-segment "equal keys straddle the median"
+  uint32_t *perm;    // tree position -> working id
+};
+
+Shape make_shape(int64_t n, int leaf);
+size_t sort_temp_bytes(uint32_t n);
+hipError_t launch_shape(const TreeDev &t, const Shape &sh, hipStream_t s);
+hipError_t launch_build(const Shape &sh, const StateDev &in, const StateDev &out,
+                        const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
+                        hipStream_t s);
+hipError_t launch_traverse(int precision, bool count, const TreeDev &t,
+                           const double4 *pos64, const float4 *pos32, uint32_t n,
+                           double theta, int leaf, double *ax, double *ay, double *az,
+                           unsigned long long *counters, hipStream_t s);
+hipError_t launch_kick_drift(const StateDev &s, const double *ax, const double *ay,
+                             const double *az, uint32_t n, double dt, int *bad,
+                             hipStream_t st);
+hipError_t launch_to_original(const uint32_t *oid, const double *a0, const double *a1,
+                              const double *a2, double *o0, double *o1, double *o2,
+                              uint32_t n, hipStream_t st);
+hipError_t launch_inverse(const uint32_t *oid, uint32_t *inv, uint32_t n, hipStream_t st);
+hipError_t launch_iota(uint32_t *v, uint32_t n, hipStream_t st);
+hipError_t launch_sanitize(const StateDev &s, uint32_t n, int *bad, hipStream_t st);
+hipError_t launch_direct(const StateDev &s, uint32_t n, const uint32_t *wpos, uint32_t k,
+                         double *out, hipStream_t st);
+
+}  // namespace kd
