@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark: particle-steps/s of the kD-tree Barnes-Hut step on one MI355X.
+
+Workload (the headline metric's configuration): ring distribution, N = 10M
+bodies, theta = 0.3, leaf size 7, dt = 1e-3, mixed precision (FP32 far-field
+and P2P with compensated FP32 sums; FP64 state, integrator and node moments).
+Beside it, on the same bodies: the plain FP64 GPU step, the median / p99
+relative force error of the mixed step against an FP64 direct sum on 4,096
+seeded bodies, the traversal kernel's roofline from its own interaction
+counters and HIP-event durations, and the reference's C implementation timed
+on the host cores on a bounded sample.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from multilanguagekdtree_amd import kdtree as K  # noqa: E402
+
+PEAK_FP32_VALU_TFLOPS = 157.3  # MI355X published vector FP32 peak
+PEAK_FP64_VALU_TFLOPS = 78.6   # MI355X published vector FP64 peak
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_gpu(bodies, n, precision, steps, warmup, theta, leaf, dt):
+    with K.Simulation(n, theta=theta, dt=dt, leaf_size=leaf, precision=precision,
+                      profile=True) as s:
+        s.upload(bodies)
+        if warmup:
+            s.step(warmup)
+        s.sync()
+        s.reset_timings()
+        t0 = time.perf_counter()
+        s.step(steps)
+        s.sync()
+        t1 = time.perf_counter()
+        tm = s.timings()
+        # interaction counts of one more step (same tree shape, positions
+        # moved by one dt): T node tests, A accepted nodes, P pairs
+        s.set_params(count_interactions=1, profile=0)
+        s.step(1)
+        cnt = s.counters()
+    wall = t1 - t0
+    return dict(wall_s=wall, ms_per_step=1e3 * wall / steps,
+                rate=n * steps / wall,
+                build_ms=tm["build_ms"] / steps, traverse_ms=tm["traverse_ms"] / steps,
+                kick_drift_ms=tm["kick_drift_ms"] / steps,
+                tests=int(cnt[0]), m2p=int(cnt[1]), p2p=int(cnt[2]))
+
+
+def force_error(bodies, n, theta, leaf, precision, k=4096, seed=2024):
+    with K.Simulation(n, theta=theta, leaf_size=leaf, precision=precision) as s:
+        s.upload(bodies)
+        a = s.accel()
+        idx = np.sort(np.random.default_rng(seed).choice(n, size=min(k, n), replace=False))
+        ref, _ = s.direct_sum(idx)
+    rel = np.linalg.norm(a[idx] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    return float(np.median(rel)), float(np.percentile(rel, 99)), float(rel.max())
+
+
+def cpu_baseline(bodies, n, sample, threads, dt):
+    """The reference's own C implementation (Parallel/CVersion, OpenMP) timed
+    on the host: full sequential build, calc_accel over a random sample of
+    bodies in parallel, kick-drift over all bodies; scaled to one step."""
+    import oracle
+    rng = np.random.default_rng(7)
+    idx = np.sort(rng.choice(n, size=min(sample, n), replace=False)).astype(np.int64)
+    R = oracle.ref_lib("par")
+    f = {k: np.ascontiguousarray(bodies[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    if R is not None:
+        import ctypes as C
+        d = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        out = np.empty(3 * len(idx))
+        t = np.zeros(3)
+        R.ref_cpu_sample(n, d(f["x"]), d(f["y"]), d(f["z"]), d(f["vx"]), d(f["vy"]),
+                         d(f["vz"]), d(f["m"]), len(idx),
+                         idx.ctypes.data_as(C.POINTER(C.c_int64)), threads, dt, d(out), d(t))
+        kind, cores = "reference", threads
+        t_build, t_acc, t_int = t
+        what = (f"reference C (Parallel/CVersion kdtree.c, gcc -O2, OpenMP {threads} threads): "
+                f"sequential build_tree of all {n} bodies {t_build:.2f} s, calc_accel of "
+                f"{len(idx)} random bodies {t_acc:.2f} s, kick-drift of all {t_int:.3f} s")
+    else:
+        # the oracle's port (single thread): build + sampled accel
+        t0 = time.perf_counter()
+        oracle.build_tree(f, leaf=7)
+        t1 = time.perf_counter()
+        oracle.accel(f, idx=idx)
+        t2 = time.perf_counter()
+        t_build, t_acc, t_int = t1 - t0, (t2 - t1) - (t1 - t0), 0.0
+        kind, cores = "port", 1
+        what = f"oracle port, 1 thread: build {t_build:.2f} s, accel of {len(idx)} bodies"
+    step_s = t_build + t_acc * n / len(idx) + t_int
+    return dict(value=n / step_s, unit="particle-steps/s", cores=cores, kind=kind,
+                sample=what + f"; one step extrapolated = {step_s:.2f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=10_000_000)
+    ap.add_argument("--theta", type=float, default=0.3)
+    ap.add_argument("--leaf", type=int, default=7)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-error", action="store_true")
+    ap.add_argument("--only", choices=["mixed", "fp64", "both"], default="both")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
+              "(this build has no multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+
+    n, dt = args.n, 1e-3
+    log(f"[bench] generating ring ICs n={n} seed={args.seed}")
+    bodies = K.ring(n, seed=args.seed)
+    res = {}
+    for name, prec in (("fp64", K.KD_PREC_FP64), ("mixed", K.KD_PREC_MIXED)):
+        if args.only not in ("both", name):
+            continue
+        log(f"[bench] {name}: warmup {args.warmup}, timing {args.steps} steps")
+        res[name] = run_gpu(bodies, n, prec, args.steps, args.warmup, args.theta, args.leaf, dt)
+        log(f"[bench] {name}: {json.dumps(res[name])}")
+    main_name = "mixed" if "mixed" in res else "fp64"
+    r = res[main_name]
+
+    err = None
+    if not args.no_error:
+        log("[bench] force error vs FP64 direct sum on 4096 bodies")
+        prec = K.KD_PREC_MIXED if main_name == "mixed" else K.KD_PREC_FP64
+        med, p99, mx = force_error(bodies, n, args.theta, args.leaf, prec)
+        err = dict(median=med, p99=p99, max=mx, samples=4096)
+        log(f"[bench] error {err}")
+
+    # roofline of the dominant kernel (the traversal): algorithmic FLOPs per
+    # launch = 10 T + 9 A + 18 P over all bodies, from the kernel's counters
+    flops = 10 * r["tests"] + 9 * r["m2p"] + 18 * r["p2p"]
+    achieved = flops / (r["traverse_ms"] * 1e-3) / 1e12
+    peak = PEAK_FP32_VALU_TFLOPS if main_name == "mixed" else PEAK_FP64_VALU_TFLOPS
+    traffic = None
+    tf = ROOT / "profiles" / "traffic.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get(f"traverse_{main_name}_n{n}")
+        except Exception:
+            traffic = None
+    roofline = dict(bound="valu", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
+                    frac=round(achieved / peak, 4), traffic=traffic,
+                    kernel=f"k_traverse_{'mixed' if main_name == 'mixed' else 'f64'}",
+                    kernel_ms=round(r["traverse_ms"], 3),
+                    flops_per_launch=flops,
+                    per_particle=dict(tests=r["tests"] / n, m2p=r["m2p"] / n, p2p=r["p2p"] / n))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or (os.cpu_count() or 1)
+        log(f"[bench] CPU baseline: reference C, {threads} threads, sample {args.cpu_sample}")
+        cpu = cpu_baseline(bodies, n, args.cpu_sample, threads, dt)
+        log(f"[bench] cpu {cpu}")
+
+    out = {
+        "metric": "particle-steps/s @N=10M theta=0.3: FP32-mixed vs FP64 GPU @ median force err<=1e-6",
+        "value": r["rate"],
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if main_name == "mixed" else "f64",
+        "data": "synthetic (seeded ring ICs generated in-process)",
+        "config": {"workload": f"kd-tree Barnes-Hut step, ring N={n}, theta={args.theta}, "
+                               f"leaf {args.leaf}, dt {dt}, {main_name} precision",
+                   "n": n, "theta": args.theta, "leaf_size": args.leaf, "dt": dt,
+                   "precision": main_name, "seed": args.seed},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "phases_ms": {k: round(r[k], 3) for k in ("build_ms", "traverse_ms", "kick_drift_ms")},
+        "force_error": err,
+    }
+    if "fp64" in res and "mixed" in res:
+        out["fp64_gpu"] = {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms_per_step"],
+                           "phases_ms": {k: round(res["fp64"][k], 3)
+                                         for k in ("build_ms", "traverse_ms", "kick_drift_ms")}}
+        out["speedup_mixed_vs_fp64"] = res["mixed"]["rate"] / res["fp64"]["rate"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -51,7 +51,8 @@ typedef struct kd_params {
   int32_t leaf_size; /* MAX_PARTS: 7 (Sequential :6) or 8 (Parallel :14); 2..32 */
   int32_t precision; /* KD_PREC_* */
   int32_t count_interactions; /* nonzero: kernels also count tests/M2P/P2P */
-  int32_t reserved;
+  int32_t profile;   /* nonzero: time build / traversal / kick-drift with HIP
+                        events on the context's stream (see kd_timings) */
 } kd_params;
 
 typedef struct kd_ctx kd_ctx;
@@ -134,6 +135,13 @@ int kd_sync(kd_ctx *ctx);
 /* Node tests, accepted far-field interactions and P2P pairs summed over the
  * bodies of the last traversal (needs params.count_interactions). */
 int kd_counters(kd_ctx *ctx, int64_t out[3]);
+
+/* With params.profile set, kd_step accumulates per phase the HIP-event time
+ * on the context's stream: out = {build ms, traversal ms, kick-drift ms,
+ * steps timed}.  kd_reset_timings zeroes the sums.  No counterpart in the
+ * reference (which times whole runs, Parallel/RustVersion/src/main.rs:25-31). */
+int kd_timings(kd_ctx *ctx, double out[4]);
+int kd_reset_timings(kd_ctx *ctx);
 
 /* FP64 O(N) direct sum (calc_pp_accel form, array_particle.rs:67-76) for k
  * sample bodies given by original index, over all bodies; abs_* (nullable)

@@ -42,7 +42,7 @@ class KdParams(C.Structure):
         ("leaf_size", C.c_int32),
         ("precision", C.c_int32),
         ("count_interactions", C.c_int32),
-        ("reserved", C.c_int32),
+        ("profile", C.c_int32),
     ]
 
 
@@ -87,6 +87,8 @@ SIGNATURES = {
     "kd_step": (C.c_int, [C.c_void_p, C.c_int64]),
     "kd_sync": (C.c_int, [C.c_void_p]),
     "kd_counters": (C.c_int, [C.c_void_p, _i64]),
+    "kd_timings": (C.c_int, [C.c_void_p, _d]),
+    "kd_reset_timings": (C.c_int, [C.c_void_p]),
     "kd_direct_sum": (C.c_int, [C.c_void_p, C.c_int64, _i64] + [_d] * 6),
     "kd_ic_ring": (C.c_int, [C.c_uint64, C.c_int64] + [_d] * 7),
     "kd_ic_circular_orbits": (C.c_int, [C.c_uint64, C.c_int64] + [_d] * 7),

@@ -65,6 +65,12 @@ struct kd_ctx {
   int *bad = nullptr;
   std::vector<void *> allocs;
   std::vector<void *> node_allocs;
+  // phase timing (params.profile): 4 events per step, drained every PROF_CHUNK steps
+  static constexpr int PROF_CHUNK = 32;
+  std::vector<hipEvent_t> ev;
+  int ev_used = 0;  // steps recorded in the current chunk
+  double t_ms[3] = {0.0, 0.0, 0.0};
+  int64_t t_steps = 0;
 };
 
 template <typename T>
@@ -161,6 +167,7 @@ int kd_destroy(kd_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (void *p : c->allocs) (void)hipFree(p);
   for (void *p : c->node_allocs) (void)hipFree(p);
+  for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return KD_OK;
@@ -394,20 +401,55 @@ int kd_accel_device(kd_ctx *c, double *ax, double *ay, double *az) {
   return accel_impl(c, ax, ay, az, hipMemcpyDeviceToDevice);
 }
 
+// Fold the recorded event intervals of the current chunk into the sums.
+static int drain_timings(kd_ctx *c) {
+  if (c->ev_used == 0) return KD_OK;
+  HIP_TRY(hipEventSynchronize(c->ev[4 * (c->ev_used - 1) + 3]));
+  for (int s = 0; s < c->ev_used; ++s) {
+    for (int p = 0; p < 3; ++p) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, c->ev[4 * s + p], c->ev[4 * s + p + 1]));
+      c->t_ms[p] += ms;
+    }
+  }
+  c->t_steps += c->ev_used;
+  c->ev_used = 0;
+  return KD_OK;
+}
+
+static int record(kd_ctx *c, int phase) {
+  if (!c->params.profile) return KD_OK;
+  if (c->ev.empty()) {
+    c->ev.resize(4 * kd_ctx::PROF_CHUNK);
+    for (auto &e : c->ev) HIP_TRY(hipEventCreate(&e));
+  }
+  if (phase == 0 && c->ev_used == kd_ctx::PROF_CHUNK) {
+    int rc = drain_timings(c);
+    if (rc) return rc;
+  }
+  HIP_TRY(hipEventRecord(c->ev[4 * c->ev_used + phase], c->stream));
+  if (phase == 3) ++c->ev_used;
+  return KD_OK;
+}
+
 int kd_step(kd_ctx *c, int64_t steps) {
   if (!c) return fail(KD_ERR_ARG, "ctx is NULL");
   if (!c->has_state) return fail(KD_ERR_STATE, "no bodies uploaded");
   if (steps < 0) return fail(KD_ERR_ARG, "steps must be >= 0");
   HIP_TRY(hipSetDevice(c->device));
   HIP_TRY(hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
+  int rc;
   for (int64_t s = 0; s < steps; ++s) {
-    int rc = build_impl(c);
-    if (rc) return rc;
-    rc = traverse_impl(c);
-    if (rc) return rc;
+    if ((rc = record(c, 0))) return rc;
+    if ((rc = build_impl(c))) return rc;
+    if ((rc = record(c, 1))) return rc;
+    if ((rc = traverse_impl(c))) return rc;
+    if ((rc = record(c, 2))) return rc;
     HIP_TRY(launch_kick_drift(c->st[c->cur], c->acc[0], c->acc[1], c->acc[2], (uint32_t)c->n,
                               c->params.dt, c->bad, c->stream));
+    if ((rc = record(c, 3))) return rc;
   }
+  if (c->params.profile && (rc = drain_timings(c))) return rc;
   int bad = 0;
   HIP_TRY(hipMemcpyAsync(&bad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -429,6 +471,26 @@ int kd_counters(kd_ctx *c, int64_t out[3]) {
   HIP_TRY(hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int i = 0; i < 3; ++i) out[i] = (int64_t)h[i];
+  return KD_OK;
+}
+
+int kd_timings(kd_ctx *c, double out[4]) {
+  if (!c || !out) return fail(KD_ERR_ARG, "NULL argument");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = drain_timings(c);
+  if (rc) return rc;
+  for (int p = 0; p < 3; ++p) out[p] = c->t_ms[p];
+  out[3] = (double)c->t_steps;
+  return KD_OK;
+}
+
+int kd_reset_timings(kd_ctx *c) {
+  if (!c) return fail(KD_ERR_ARG, "ctx is NULL");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = drain_timings(c);
+  if (rc) return rc;
+  c->t_ms[0] = c->t_ms[1] = c->t_ms[2] = 0.0;
+  c->t_steps = 0;
   return KD_OK;
 }
 

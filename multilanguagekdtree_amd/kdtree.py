@@ -45,12 +45,13 @@ class Simulation:
 
     def __init__(self, capacity: int, theta: float = 0.3, dt: float = 1e-3,
                  leaf_size: int = 7, precision: int = KD_PREC_FP64,
-                 count_interactions: bool = False):
+                 count_interactions: bool = False, profile: bool = False):
         L = _lib.lib()
         self._L = L
         p = _lib.default_params(theta=theta, dt=dt, leaf_size=leaf_size,
                                 precision=precision,
-                                count_interactions=int(count_interactions))
+                                count_interactions=int(count_interactions),
+                                profile=int(profile))
         ctx = C.c_void_p()
         check(L.kd_create(capacity, C.byref(p), C.byref(ctx)))
         self._ctx = ctx
@@ -125,6 +126,16 @@ class Simulation:
         out = np.zeros(3, dtype=np.int64)
         check(self._L.kd_counters(self._ctx, iptr(out)))
         return out
+
+    def timings(self) -> dict:
+        """Summed HIP-event times (ms) of kd_step phases since the last reset."""
+        out = np.zeros(4)
+        check(self._L.kd_timings(self._ctx, dptr(out)))
+        return dict(build_ms=out[0], traverse_ms=out[1], kick_drift_ms=out[2],
+                    steps=int(out[3]))
+
+    def reset_timings(self):
+        check(self._L.kd_reset_timings(self._ctx))
 
     def direct_sum(self, idx) -> tuple[np.ndarray, np.ndarray]:
         idx = np.ascontiguousarray(idx, dtype=np.int64)
