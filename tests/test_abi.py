@@ -1,0 +1,93 @@
+"""The C ABI library (CPU only): loads, exports every symbol declared in
+include/kdtree_hip.h, and its host-only entry points behave."""
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from multilanguagekdtree_amd import _lib
+from multilanguagekdtree_amd import kdtree as K
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "kdtree_hip.h"
+
+
+def declared():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(kd_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared()
+    assert len(names) >= 25
+    for n in names:
+        assert hasattr(L, n), n
+    # and the Python binding covers the same set
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def test_version_and_defaults():
+    L = _lib.lib()
+    assert b"gfx950" in L.kd_version()
+    p = _lib.default_params()
+    assert (p.theta, p.dt, p.leaf_size, p.precision) == (0.3, 1e-3, 7, 0)
+
+
+def test_nodes_needed_matches_reference_formulas(orc):
+    for n in (1, 2, 7, 8, 9, 12, 100, 1000, 12345, 10**6):
+        for leaf in (2, 7, 8, 32):
+            for layout in (0, 1):
+                assert K.allocate_node_vec(n, leaf, layout) == orc.nodes_needed(n, leaf, layout)
+
+
+def test_ring_initial_conditions():
+    b = K.ring(20000, seed=5)
+    r = np.hypot(b["x"], b["y"])
+    assert b["m"][0] == 1.0 and b["x"][0] == 0.0 and b["vx"][0] == 0.0
+    assert np.all((r[1:] >= 1.0) & (r[1:] < 2.0 + 1e-15))
+    assert np.all(b["m"][1:] == 1e-14)
+    assert abs(np.std(b["z"][1:]) - 1e-3) < 5e-5
+    v = np.hypot(b["vx"], b["vy"])
+    np.testing.assert_allclose(v[1:], np.sqrt(1.0 / r[1:]), rtol=1e-12)
+    assert np.allclose(b["x"][1:] * b["vx"][1:] + b["y"][1:] * b["vy"][1:], 0.0, atol=1e-12)
+    b2 = K.ring(20000, seed=5)
+    assert all(np.array_equal(b[k], b2[k]) for k in b)
+
+
+def test_circular_orbits_mirror_reference_formula():
+    # array_particle.rs:19-44: n+1 bodies, d = 0.1 + i*5/n, v = sqrt(1/d), planar
+    n = 1000
+    b = K.circular_orbits(n, seed=1)
+    assert len(b["x"]) == n + 1
+    d = np.hypot(b["x"][1:], b["y"][1:])
+    np.testing.assert_allclose(d, 0.1 + np.arange(n) * 5.0 / n, rtol=1e-14)
+    assert np.all(b["z"] == 0.0) and np.all(b["vz"] == 0.0)
+    np.testing.assert_allclose(np.hypot(b["vx"][1:], b["vy"][1:]), np.sqrt(1.0 / d), rtol=1e-14)
+
+
+def test_two_bodies():
+    b = K.two_bodies()
+    assert b["x"].tolist() == [0.0, 1.0] and b["vy"].tolist() == [0.0, 1.0]
+    assert b["m"].tolist() == [1.0, 1e-20]
+
+
+def test_create_without_device_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(_lib.KdError) as e:
+        K.Simulation(100)
+    assert e.value.code in (_lib.KD_ERR_NODEVICE, _lib.KD_ERR_HIP)
+
+
+def test_bad_arguments_rejected_on_host():
+    L = _lib.lib()
+    import ctypes as C
+    ctx = C.c_void_p()
+    assert L.kd_create(0, None, C.byref(ctx)) == _lib.KD_ERR_ARG
+    p = _lib.default_params(leaf_size=1)
+    assert L.kd_create(10, C.byref(p), C.byref(ctx)) == _lib.KD_ERR_ARG
+    assert b"leaf_size" in L.kd_last_error()
+    assert L.kd_step(None, 1) == _lib.KD_ERR_ARG

@@ -161,3 +161,39 @@ def test_errors():
         with pytest.raises(_lib.KdError) as e:
             s.set_params(leaf_size=1)
         assert e.value.code == _lib.KD_ERR_ARG
+
+
+# ---- golden vectors of the reference's own C build (tests/golden) ----------
+
+def _gold():
+    from pathlib import Path
+    return np.load(Path(__file__).resolve().parent / "golden" / "config0.npz", allow_pickle=False)
+
+
+def test_golden_tree_config0():
+    g = _gold()
+    b = {k: g[k] for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    t = gpu_tree(b)
+    nn = len(g["tree_num_parts"])
+    np.testing.assert_array_equal(t.num_parts[:nn], g["tree_num_parts"])
+    internal = g["tree_num_parts"] == 0
+    for i in np.nonzero(~internal)[0]:
+        np.testing.assert_array_equal(t.leaf_parts(i), g["tree_parts"][i][:g["tree_num_parts"][i]])
+    for k in ("split_dim", "left", "right"):
+        np.testing.assert_array_equal(getattr(t, k)[:nn][internal], g[f"tree_{k}"][internal])
+    for k in ("split_val", "size"):
+        assert np.array_equal(getattr(t, k)[:nn][internal].view(np.int64),
+                              g[f"tree_{k}"][internal].view(np.int64)), k
+
+
+def test_golden_accel_and_trajectory_config0():
+    g = _gold()
+    b = {k: g[k] for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    a = K.calc_accel(b)
+    ref = g["acc_step0"].T
+    da = np.linalg.norm(a - ref, axis=1)
+    assert np.all(da[1:] <= ACC_TOL * np.linalg.norm(ref[1:], axis=1))
+    out = K.simple_sim(b, dt=1e-3, steps=10)
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        r = g[f"final_{k}"]
+        assert np.abs(out[k] - r).max() <= 1e-12 * np.abs(r).max(), k
