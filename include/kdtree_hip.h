@@ -132,9 +132,19 @@ int kd_step(kd_ctx *ctx, int64_t steps);
 /* Wait for the context's stream. */
 int kd_sync(kd_ctx *ctx);
 
-/* Node tests, accepted far-field interactions and P2P pairs summed over the
- * bodies of the last traversal (needs params.count_interactions). */
-int kd_counters(kd_ctx *ctx, int64_t out[3]);
+/* Counters of the last traversal (needs params.count_interactions):
+ * out[0] node opening tests, out[1] accepted far-field interactions, out[2]
+ * P2P pairs, each summed over bodies (the reference's per-body work); out[3]
+ * node visits and out[4] leaf visits summed over wavefronts (the work the
+ * 64-wide wavefronts actually issue). */
+int kd_counters(kd_ctx *ctx, int64_t out[5]);
+
+/* Launch-shape knobs of the kernels (no counterpart in the reference):
+ * "leaf_predicated" 0/1 -- compute whole leaf batches with predication
+ * instead of branching per body; "fp64_batched_walk" 0/1 -- run the FP64
+ * path on the batched walk of the mixed paths.  Unknown keys return
+ * KD_ERR_ARG. */
+int kd_set_tuning(kd_ctx *ctx, const char *key, int64_t value);
 
 /* With params.profile set, kd_step accumulates per phase the HIP-event time
  * on the context's stream: out = {build ms, traversal ms, kick-drift ms,

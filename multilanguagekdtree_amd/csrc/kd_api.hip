@@ -71,6 +71,7 @@ struct kd_ctx {
   int ev_used = 0;  // steps recorded in the current chunk
   double t_ms[3] = {0.0, 0.0, 0.0};
   int64_t t_steps = 0;
+  int traverse_variant = 0;  // bit 1: predicated leaf batches; bit 2: FP64 on the batched walk
 };
 
 template <typename T>
@@ -206,14 +207,16 @@ static int create_impl(kd_ctx *c, int64_t cap) {
   ALLOC(c->allocs, c->b.loc, N);
   ALLOC(c->allocs, c->b.perm, N);
   ALLOC(c->allocs, c->inv, N);
-  ALLOC(c->allocs, c->pos64, N);
-  ALLOC(c->allocs, c->pos32, N);
+  ALLOC(c->allocs, c->pos64, N + BODY_PAD);
+  ALLOC(c->allocs, c->pos32, N + BODY_PAD);
+  HIP_TRY(hipMemsetAsync(c->pos64, 0, sizeof(double4) * (N + BODY_PAD), c->stream));
+  HIP_TRY(hipMemsetAsync(c->pos32, 0, sizeof(float4) * (N + BODY_PAD), c->stream));
   size_t max_seg = 2 * (N / SUB_MAX) + 64;
   ALLOC(c->allocs, c->b.cnt, 3 * (N / LVL_TILE + max_seg + 64));
   ALLOC(c->allocs, c->b.tieflag, max_seg);
   c->b.sort_tmp_bytes = sort_temp_bytes((uint32_t)cap);
   ALLOC(c->allocs, *(uint8_t **)&c->b.sort_tmp, c->b.sort_tmp_bytes);
-  ALLOC(c->allocs, c->counters, 4);
+  ALLOC(c->allocs, c->counters, 8);
   ALLOC(c->allocs, c->bad, 4);
   return KD_OK;
 }
@@ -349,7 +352,7 @@ int kd_download_device(kd_ctx *c, double *x, double *y, double *z, double *vx, d
 static int build_impl(kd_ctx *c) {
   int nxt = c->cur ^ 1;
   HIP_TRY(launch_build(c->shape, c->st[c->cur], c->st[nxt], c->tree, c->b, c->pos64, c->pos32,
-                       c->stream));
+                       c->params.theta, c->stream));
   c->cur = nxt;
   c->has_tree = true;
   return KD_OK;
@@ -357,8 +360,9 @@ static int build_impl(kd_ctx *c) {
 
 static int traverse_impl(kd_ctx *c) {
   bool count = c->params.count_interactions != 0;
-  if (count) HIP_TRY(hipMemsetAsync(c->counters, 0, 4 * sizeof(unsigned long long), c->stream));
-  HIP_TRY(launch_traverse(c->params.precision, count, c->tree, c->pos64, c->pos32,
+  if (count) HIP_TRY(hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream));
+  HIP_TRY(launch_traverse(c->params.precision, count, c->traverse_variant, c->tree, c->pos64,
+                          c->pos32,
                           (uint32_t)c->n, c->params.theta, c->params.leaf_size, c->acc[0],
                           c->acc[1], c->acc[2], c->counters, c->stream));
   return KD_OK;
@@ -464,14 +468,27 @@ int kd_sync(kd_ctx *c) {
   return KD_OK;
 }
 
-int kd_counters(kd_ctx *c, int64_t out[3]) {
+int kd_counters(kd_ctx *c, int64_t out[5]) {
   if (!c || !out) return fail(KD_ERR_ARG, "NULL argument");
   HIP_TRY(hipSetDevice(c->device));
-  unsigned long long h[4] = {0, 0, 0, 0};
+  unsigned long long h[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   HIP_TRY(hipMemcpyAsync(h, c->counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  for (int i = 0; i < 3; ++i) out[i] = (int64_t)h[i];
+  for (int i = 0; i < 5; ++i) out[i] = (int64_t)h[i];
   return KD_OK;
+}
+
+int kd_set_tuning(kd_ctx *c, const char *key, int64_t value) {
+  if (!c || !key) return fail(KD_ERR_ARG, "NULL argument");
+  if (strcmp(key, "leaf_predicated") == 0) {
+    c->traverse_variant = value ? (c->traverse_variant | 2) : (c->traverse_variant & ~2);
+    return KD_OK;
+  }
+  if (strcmp(key, "fp64_batched_walk") == 0) {
+    c->traverse_variant = value ? (c->traverse_variant | 4) : (c->traverse_variant & ~4);
+    return KD_OK;
+  }
+  return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
 int kd_timings(kd_ctx *c, double out[4]) {

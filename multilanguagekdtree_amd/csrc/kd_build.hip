@@ -419,38 +419,53 @@ struct SubArgs {
   float4 *pos32;
   uint32_t level;
   int leaf;
+  double theta2;
 };
+
+// Heap index (1-based) of the node after the subtree of heap slot h (0-based).
+__device__ __forceinline__ uint32_t skip_of(uint32_t h) {
+  uint32_t x = h + 1;
+  x >>= __builtin_ctz(~x);
+  return x == 0 ? 0u : x + 1;
+}
 
 __device__ void write_leaf_rec(const TreeDev &t, uint32_t h, uint32_t start,
                                uint32_t count) {
+  const uint32_t skip = skip_of(h);
   NodeRec64 r;
   r.cx = r.cy = r.cz = r.m = r.size2 = 0.0;
   r.start = start;
   r.count = count;
-  r.pad0 = r.pad1 = 0.0;
+  r.skip = skip;
+  r.pad0 = 0;
+  r.pad1 = 0.0;
   t.rec64[h] = r;
   NodeRec32 q;
   q.cx = q.cy = q.cz = q.m = q.size2 = 0.f;
   q.start = start;
   q.count = count;
-  q.pad = 0.f;
+  q.skip = skip;
   t.rec32[h] = q;
   NodeRecD dd;
   dd.cx = dd.cy = dd.cz = 0.0;
   dd.m = dd.size2 = 0.f;
   dd.start = start;
   dd.count = count;
-  dd.pad0 = dd.pad1 = 0.0;
+  dd.skip = skip;
+  dd.pad0 = 0;
+  dd.pad1 = 0.0;
   t.recd[h] = dd;
 }
 
 __device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
-                                   uint32_t count) {
+                                   uint32_t count, double theta2) {
 #pragma clang fp contract(off)
   double m = t.mm[h];
   double cx = t.mx[h] / m, cy = t.my[h] / m, cz = t.mz[h] / m;
   double sz = t.size[h];
   double s2 = sz * sz;
+  float open2 = (float)(s2 / theta2);  // theta = 0: +inf (or NaN): never accepted
+  const uint32_t skip = skip_of(h);
   NodeRec64 r;
   r.cx = cx;
   r.cy = cy;
@@ -459,27 +474,31 @@ __device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
   r.size2 = s2;
   r.start = start;
   r.count = count;
-  r.pad0 = r.pad1 = 0.0;
+  r.skip = skip;
+  r.pad0 = 0;
+  r.pad1 = 0.0;
   t.rec64[h] = r;
   NodeRec32 q;
   q.cx = (float)cx;
   q.cy = (float)cy;
   q.cz = (float)cz;
   q.m = (float)m;
-  q.size2 = (float)s2;
+  q.size2 = open2;
   q.start = start;
   q.count = count;
-  q.pad = 0.f;
+  q.skip = skip;
   t.rec32[h] = q;
   NodeRecD dd;
   dd.cx = cx;
   dd.cy = cy;
   dd.cz = cz;
   dd.m = (float)m;
-  dd.size2 = (float)s2;
+  dd.size2 = open2;
   dd.start = start;
   dd.count = count;
-  dd.pad0 = dd.pad1 = 0.0;
+  dd.skip = skip;
+  dd.pad0 = 0;
+  dd.pad1 = 0.0;
   t.recd[h] = dd;
 }
 
@@ -731,7 +750,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
       a.t.mx[h] = a.t.mx[l] + a.t.mx[r];
       a.t.my[h] = a.t.my[l] + a.t.my[r];
       a.t.mz[h] = a.t.mz[l] + a.t.mz[r];
-      write_internal_rec(a.t, h, s0 + s, cnt);
+      write_internal_rec(a.t, h, s0 + s, cnt, a.theta2);
     }
     __threadfence_block();
     __syncthreads();
@@ -739,7 +758,8 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
 }
 
 // Moments and records of the device-wide levels [0, level), bottom-up.
-__global__ __launch_bounds__(1024) void k_moments_top(TreeDev t, uint32_t level) {
+__global__ __launch_bounds__(1024) void k_moments_top(TreeDev t, uint32_t level,
+                                                      double theta2) {
 #pragma clang fp contract(off)
   for (int lev = (int)level - 1; lev >= 0; --lev) {
     uint32_t nseg = 1u << lev;
@@ -750,7 +770,7 @@ __global__ __launch_bounds__(1024) void k_moments_top(TreeDev t, uint32_t level)
       t.mx[h] = t.mx[l] + t.mx[r];
       t.my[h] = t.my[l] + t.my[r];
       t.mz[h] = t.mz[l] + t.mz[r];
-      write_internal_rec(t, h, t.seg_start[h], t.seg_count[h]);
+      write_internal_rec(t, h, t.seg_start[h], t.seg_count[h], theta2);
     }
     __threadfence_block();
     __syncthreads();
@@ -800,7 +820,7 @@ hipError_t launch_shape(const TreeDev &t, const Shape &sh, hipStream_t s) {
 
 hipError_t launch_build(const Shape &sh, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64,
-                        float4 *pos32, hipStream_t s) {
+                        float4 *pos32, double theta, hipStream_t s) {
   const uint32_t n = (uint32_t)sh.n;
   const double *coord[3] = {in.x, in.y, in.z};
   hipError_t e;
@@ -842,8 +862,9 @@ hipError_t launch_build(const Shape &sh, const StateDev &in, const StateDev &out
   a.pos32 = pos32;
   a.level = sh.sub_level;
   a.leaf = sh.leaf;
+  a.theta2 = theta * theta;
   k_subtree<<<1u << sh.sub_level, SUB_THREADS, 0, s>>>(a);
-  if (sh.sub_level > 0) k_moments_top<<<1, 1024, 0, s>>>(t, sh.sub_level);
+  if (sh.sub_level > 0) k_moments_top<<<1, 1024, 0, s>>>(t, sh.sub_level, theta * theta);
   return hipGetLastError();
 }
 

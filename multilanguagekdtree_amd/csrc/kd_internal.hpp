@@ -20,23 +20,30 @@ constexpr int LVL_ITEMS = 8;
 constexpr int LVL_TILE = LVL_THREADS * LVL_ITEMS;  // 2048
 
 constexpr int MAX_LEAF = 32;
+// Tree-ordered body arrays carry this many padding entries so that batched
+// leaf loads (up to 8 bodies) never leave the allocation.
+constexpr int BODY_PAD = 32;
 
 // Node record read by the FP64 traversal: 64 bytes, one scalar load.
 // Internal nodes: centre of mass, mass, size*size, and the particle range of
-// the node.  Leaves: the particle range only (count <= leaf size).
+// the node.  Leaves: the particle range only (count <= leaf size).  skip is
+// the 1-based heap index of the node that follows this subtree in depth-first
+// order (0 after the last one).
 struct alignas(64) NodeRec64 {
   double cx, cy, cz, m;
   double size2;
   uint32_t start, count;
-  double pad0, pad1;
+  uint32_t skip, pad0;
+  double pad1;
 };
 
-// Node record read by the mixed-precision traversal: 32 bytes.
+// Node record read by the mixed-precision traversal: 32 bytes.  size2 holds
+// the opening threshold size^2 / theta^2 (accept iff size2 < d2).
 struct alignas(32) NodeRec32 {
   float cx, cy, cz, m;
   float size2;
   uint32_t start, count;
-  float pad;
+  uint32_t skip;
 };
 
 // Node record for the mixed traversal with FP64 coordinate differences.
@@ -44,7 +51,8 @@ struct alignas(64) NodeRecD {
   double cx, cy, cz;
   float m, size2;
   uint32_t start, count;
-  double pad0, pad1;
+  uint32_t skip, pad0;
+  double pad1;
 };
 
 // Device-resident tree of one system in heap order: node h has children
@@ -93,8 +101,8 @@ size_t sort_temp_bytes(uint32_t n);
 hipError_t launch_shape(const TreeDev &t, const Shape &sh, hipStream_t s);
 hipError_t launch_build(const Shape &sh, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
-                        hipStream_t s);
-hipError_t launch_traverse(int precision, bool count, const TreeDev &t,
+                        double theta, hipStream_t s);
+hipError_t launch_traverse(int precision, bool count, int variant, const TreeDev &t,
                            const double4 *pos64, const float4 *pos32, uint32_t n,
                            double theta, int leaf, double *ax, double *ay, double *az,
                            unsigned long long *counters, hipStream_t s);

@@ -5,21 +5,26 @@
 // (1-based) is wave-uniform, so node records and leaf bodies arrive through
 // scalar loads; descending goes to 2h, finishing a subtree goes to
 // (h >> trailing_ones(h)) + 1.  Each lane keeps the depth D at which it last
-// accepted a node (or INT_MAX while it keeps opening): a lane takes part in
-// node h iff depth(h) <= D, so every lane sees exactly the nodes and leaves
-// its own recursive traversal (Sequential/RustVersion/src/array_kd_tree.rs:
-// 140-172) would see.  The wave descends when any active lane opens.
+// accepted a node (INT_MAX while it keeps opening): a lane takes part in node
+// h iff depth(h) <= D, so every lane sees exactly the nodes and leaves its
+// own recursive traversal (Sequential/RustVersion/src/array_kd_tree.rs:
+// 140-172) sees.  The wave descends when any active lane opens.
 //
-// FP64 kernel: the reference arithmetic, unfused (accept iff
-// size*size < theta^2*d2, M2P -m/(d2*sqrt(d2)), P2P -m/(dist*dist*dist)).
-// Mixed kernels: FP32 arithmetic with v_rsq_f32 and compensated (Kahan) FP32
-// accumulation; FP64 results handed to the FP64 integrator.
+// Arithmetic policies:
+//   F64   the reference arithmetic, unfused: accept iff size*size <
+//         (theta*theta)*d2, M2P -m/(d2*sqrt(d2)), P2P -m/(dist*dist*dist),
+//         FP64 sums (array_kd_tree.rs:600-613, array_particle.rs:67-76);
+//   F32   FP32 coordinates, v_rsq_f32, opening threshold size^2/theta^2
+//         baked into the record, blocked compensated (Kahan) FP32 sums:
+//         terms accumulate in a plain FP32 partial that is folded into the
+//         (sum, compensation) pair after every leaf;
+//   D64   as F32 but the coordinate difference is taken in FP64.
 #include "kd_internal.hpp"
 
 namespace kd {
 
 struct Counters {
-  unsigned long long *c;  // [0] node tests, [1] M2P, [2] P2P
+  unsigned long long *c;  // node tests, M2P, P2P, wave node visits, wave leaf visits
 };
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
@@ -28,23 +33,22 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
   return v;
 }
 
-template <bool COUNT>
-__device__ __forceinline__ void flush_counters(Counters cnt, unsigned long long t,
-                                               unsigned long long a,
-                                               unsigned long long p) {
-  if (!COUNT) return;
-  t = wave_sum_u64(t);
-  a = wave_sum_u64(a);
-  p = wave_sum_u64(p);
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&cnt.c[0], t);
-    atomicAdd(&cnt.c[1], a);
-    atomicAdd(&cnt.c[2], p);
-  }
-}
-
 __device__ __forceinline__ uint32_t uni(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
+}
+
+// One node record as a single wide value (a vector of 32-bit words), so that
+// the scalar unit fetches it with one load instead of per-field loads.
+// nodes_m1 points one record before the first, so the 1-based heap index h
+// is a plain 32-bit byte offset h*sizeof(Rec).
+template <class Rec>
+__device__ __forceinline__ Rec load_rec(const Rec *__restrict__ nodes_m1, uint32_t h) {
+  typedef uint32_t words __attribute__((ext_vector_type(sizeof(Rec) / 4)));
+  const char *base = reinterpret_cast<const char *>(nodes_m1);
+  const words w = *reinterpret_cast<const words *>(base + (h * (uint32_t)sizeof(Rec)));
+  Rec r;
+  __builtin_memcpy(&r, &w, sizeof(Rec));
+  return r;
 }
 
 // Advance past the subtree of h; returns 0 when the root is finished.
@@ -55,30 +59,312 @@ __device__ __forceinline__ uint32_t heap_next(uint32_t h, int &depth) {
   return h == 0 ? 0 : h + 1;
 }
 
+// ---------------------------------------------------------------------------
+// arithmetic policies
+// ---------------------------------------------------------------------------
+
+struct F64 {
+  static constexpr int CHUNK = 4;  // leaf bodies per batch of scalar loads
+  using Rec = NodeRec64;
+  using Body = double4;
+  using Self = double4;
+  struct Acc {
+    double s[3] = {0.0, 0.0, 0.0};
+  };
+  struct Diff {
+    double dx, dy, dz, d2;
+  };
+  __device__ static Self self(const Body &b) { return b; }
+  __device__ static bool test(const Self &p, const Rec &r, double theta2, Diff &f) {
+#pragma clang fp contract(off)
+    f.dx = p.x - r.cx;
+    f.dy = p.y - r.cy;
+    f.dz = p.z - r.cz;
+    f.d2 = f.dx * f.dx + f.dy * f.dy + f.dz * f.dz;
+    return r.size2 < theta2 * f.d2;
+  }
+  __device__ static void m2p(Acc &a, const Rec &r, const Diff &f) {
+#pragma clang fp contract(off)
+    double dist = sqrt(f.d2);
+    double magi = -r.m / (f.d2 * dist);
+    a.s[0] += f.dx * magi;
+    a.s[1] += f.dy * magi;
+    a.s[2] += f.dz * magi;
+  }
+  __device__ static void p2p(Acc &a, const Self &p, const Body &q, bool self_pair) {
+#pragma clang fp contract(off)
+    double dx = p.x - q.x;
+    double dy = p.y - q.y;
+    double dz = p.z - q.z;
+    double dp2 = dx * dx + dy * dy + dz * dz;
+    double dist = sqrt(dp2);
+    double magi = -q.w / (dist * dist * dist);
+    magi = self_pair ? 0.0 : magi;  // self is skipped: adds exactly +0
+    a.s[0] += magi * dx;
+    a.s[1] += magi * dy;
+    a.s[2] += magi * dz;
+  }
+  __device__ static void m2p_sel(Acc &a, const Rec &r, const Diff &f, bool take) {
+    if (take) m2p(a, r, f);
+  }
+  __device__ static void fold(Acc &) {}
+  __device__ static double result(const Acc &a, int i) { return a.s[i]; }
+};
+
+struct KahanAcc {
+  float p[3] = {0.f, 0.f, 0.f};  // partial of the current block of terms
+  float s[3] = {0.f, 0.f, 0.f};  // compensated sum
+  float c[3] = {0.f, 0.f, 0.f};  // running compensation
+};
+
+__device__ __forceinline__ void kahan_fold(KahanAcc &a) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float y = a.p[i] - a.c[i];
+    float t = a.s[i] + y;
+    a.c[i] = (t - a.s[i]) - y;
+    a.s[i] = t;
+    a.p[i] = 0.f;
+  }
+}
+
+struct F32 {
+  static constexpr int CHUNK = 8;
+  using Rec = NodeRec32;
+  using Body = float4;
+  using Self = float4;
+  using Acc = KahanAcc;
+  struct Diff {
+    float dx, dy, dz, d2;
+  };
+  __device__ static Self self(const Body &b) { return b; }
+  __device__ static bool test(const Self &p, const Rec &r, double, Diff &f) {
+    f.dx = p.x - r.cx;
+    f.dy = p.y - r.cy;
+    f.dz = p.z - r.cz;
+    f.d2 = f.dx * f.dx + f.dy * f.dy + f.dz * f.dz;
+    return r.size2 < f.d2;  // size2 holds size^2 / theta^2
+  }
+  // Every lane evaluates the term; lanes that do not accept add +0.  This
+  // keeps the walk free of divergent branches (no exec-mask save/restore).
+  __device__ static void m2p_sel(Acc &a, const Rec &r, const Diff &f, bool take) {
+    float ri = __builtin_amdgcn_rsqf(f.d2);
+    float g = -r.m * (ri * ri * ri);
+    g = take ? g : 0.f;
+    a.p[0] += g * f.dx;
+    a.p[1] += g * f.dy;
+    a.p[2] += g * f.dz;
+  }
+  __device__ static void p2p(Acc &a, const Self &p, const Body &q, bool self_pair) {
+    float dx = p.x - q.x;
+    float dy = p.y - q.y;
+    float dz = p.z - q.z;
+    float d2 = dx * dx + dy * dy + dz * dz;
+    float ri = __builtin_amdgcn_rsqf(d2);
+    float g = -q.w * (ri * ri * ri);
+    g = self_pair ? 0.f : g;
+    a.p[0] += g * dx;
+    a.p[1] += g * dy;
+    a.p[2] += g * dz;
+  }
+  __device__ static void fold(Acc &a) { kahan_fold(a); }
+  __device__ static double result(const Acc &a, int i) {
+    return ((double)a.s[i] + (double)a.p[i]) - (double)a.c[i];
+  }
+};
+
+struct D64 {
+  static constexpr int CHUNK = 4;
+  using Rec = NodeRecD;
+  using Body = double4;
+  using Self = double4;
+  using Acc = KahanAcc;
+  struct Diff {
+    float dx, dy, dz, d2;
+  };
+  __device__ static Self self(const Body &b) { return b; }
+  __device__ static bool test(const Self &p, const Rec &r, double, Diff &f) {
+    f.dx = (float)(p.x - r.cx);
+    f.dy = (float)(p.y - r.cy);
+    f.dz = (float)(p.z - r.cz);
+    f.d2 = f.dx * f.dx + f.dy * f.dy + f.dz * f.dz;
+    return r.size2 < f.d2;
+  }
+  // Every lane evaluates the term; lanes that do not accept add +0.  This
+  // keeps the walk free of divergent branches (no exec-mask save/restore).
+  __device__ static void m2p_sel(Acc &a, const Rec &r, const Diff &f, bool take) {
+    float ri = __builtin_amdgcn_rsqf(f.d2);
+    float g = -r.m * (ri * ri * ri);
+    g = take ? g : 0.f;
+    a.p[0] += g * f.dx;
+    a.p[1] += g * f.dy;
+    a.p[2] += g * f.dz;
+  }
+  __device__ static void p2p(Acc &a, const Self &p, const Body &q, bool self_pair) {
+    float dx = (float)(p.x - q.x);
+    float dy = (float)(p.y - q.y);
+    float dz = (float)(p.z - q.z);
+    float d2 = dx * dx + dy * dy + dz * dz;
+    float ri = __builtin_amdgcn_rsqf(d2);
+    float g = -(float)q.w * (ri * ri * ri);
+    g = self_pair ? 0.f : g;
+    a.p[0] += g * dx;
+    a.p[1] += g * dy;
+    a.p[2] += g * dz;
+  }
+  __device__ static void fold(Acc &a) { kahan_fold(a); }
+  __device__ static double result(const Acc &a, int i) {
+    return ((double)a.s[i] + (double)a.p[i]) - (double)a.c[i];
+  }
+};
+
+template <class P>
+struct WalkArgs {
+  const typename P::Rec *__restrict__ nodes;
+  const typename P::Body *__restrict__ bodies;  // tree order, padded by MAX_LEAF
+  uint32_t n;
+  uint32_t leaf;
+  double theta2;
+  double *__restrict__ ax;
+  double *__restrict__ ay;
+  double *__restrict__ az;
+  Counters cnt;
+};
+
+// P2P of the lane's body with n (1..B) consecutive leaf bodies starting at
+// tree position s0, entered through a switch so the scalar unit spends one
+// computed branch per batch instead of a compare-and-branch per body.  The
+// batch loads are unconditional (the body arrays carry BODY_PAD spare
+// entries); terms are added from the last body to the first.
+template <class P, int B, bool COUNT, bool PRED>
+__device__ __forceinline__ void leaf_batch(typename P::Acc &acc, const typename P::Self &p,
+                                           const typename P::Body *__restrict__ base,
+                                           uint32_t n, uint32_t s0, uint32_t me, bool active,
+                                           unsigned long long &np) {
+  typename P::Body q[B];
+#pragma unroll
+  for (int j = 0; j < B; ++j) q[j] = base[j];
+  if (!active) return;
+  if (PRED) {
+    // every body of the batch is computed; those past n contribute +0
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+      const bool use = (uint32_t)j < n;
+      P::p2p(acc, p, q[j], !use || s0 + j == me);
+      if (COUNT) np += (use && s0 + j != me) ? 1 : 0;
+    }
+    return;
+  }
+#define KD_P2P(j)                                          \
+  case (j) + 1:                                            \
+    P::p2p(acc, p, q[(j) % B], s0 + (j) == me);            \
+    if (COUNT) np += (s0 + (j) != me) ? 1 : 0;             \
+    [[fallthrough]];
+  switch (n) {
+    default:
+    if (B >= 8) { KD_P2P(7) }
+    if (B >= 7) { KD_P2P(6) }
+    if (B >= 6) { KD_P2P(5) }
+    if (B >= 5) { KD_P2P(4) }
+    KD_P2P(3)
+    KD_P2P(2)
+    KD_P2P(1)
+    KD_P2P(0)
+    case 0:
+      break;
+  }
+#undef KD_P2P
+}
+
+template <class P, bool COUNT, bool PRED>
+__global__ __launch_bounds__(256) void k_walk(const typename P::Rec *__restrict__ nodes_m1,
+                                              const typename P::Body *__restrict__ bodies,
+                                              WalkArgs<P> a) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+  if (wbase >= a.n) return;
+  const uint32_t me = wbase + lane;
+  const bool valid = me < a.n;
+  const typename P::Self p = P::self(bodies[valid ? me : wbase]);
+  typename P::Acc acc;
+  int D = valid ? 0x7fffffff : -1;
+  unsigned long long nt = 0, na = 0, np = 0, wv = 0, wl = 0;
+  uint32_t h = 1;
+  do {
+    const typename P::Rec r = load_rec(nodes_m1, uni(h));
+    const int depth = 31 - __builtin_clz(h);
+    const bool active = depth <= D;
+    if (COUNT) ++wv;
+    if (r.count <= a.leaf) {
+      if (COUNT) ++wl;
+      for (uint32_t b = 0; b < r.count; b += P::CHUNK)
+        leaf_batch<P, P::CHUNK, COUNT, PRED>(acc, p, bodies + (r.start + b),
+                                             min((uint32_t)P::CHUNK, r.count - b), r.start + b,
+                                             me, active, np);
+      if (active) P::fold(acc);
+      h = r.skip;
+    } else {
+      // the test runs on every lane (no divergence); only active lanes use it
+      typename P::Diff f;
+      const bool ok = P::test(p, r, a.theta2, f);
+      const bool take = active && ok;
+      const bool open = active && !ok;
+      P::m2p_sel(acc, r, f, take);
+      D = take ? depth : (open ? 0x7fffffff : D);
+      if (COUNT) {
+        nt += active ? 1 : 0;
+        na += take ? 1 : 0;
+      }
+      const uint64_t om = __builtin_amdgcn_ballot_w64(active) & ~__builtin_amdgcn_ballot_w64(ok);
+      h = om ? 2 * h : r.skip;
+    }
+  } while (h != 0);
+  P::fold(acc);
+  if (valid) {
+    a.ax[me] = P::result(acc, 0);
+    a.ay[me] = P::result(acc, 1);
+    a.az[me] = P::result(acc, 2);
+  }
+  if (COUNT) {
+    nt = wave_sum_u64(nt);
+    na = wave_sum_u64(na);
+    np = wave_sum_u64(np);
+    if (lane == 0) {
+      atomicAdd(&a.cnt.c[0], nt);
+      atomicAdd(&a.cnt.c[1], na);
+      atomicAdd(&a.cnt.c[2], np);
+      atomicAdd(&a.cnt.c[3], wv);
+      atomicAdd(&a.cnt.c[4], wl);
+    }
+  }
+}
+
+// The plain FP64 path keeps a simpler loop: it is VALU-bound (FP64 sqrt and
+// divide dominate), and this form issues fewer vector instructions per visit
+// than the batched walk above: leaf bodies are read one at a time under the
+// active-lane mask and the self pair is skipped by branch.
 template <bool COUNT>
-__global__ __launch_bounds__(256) void k_traverse_f64(const NodeRec64 *__restrict__ nodes,
-                                                      const double4 *__restrict__ pos,
-                                                      uint32_t n, double theta2,
-                                                      uint32_t leaf, double *__restrict__ ax,
-                                                      double *__restrict__ ay,
-                                                      double *__restrict__ az,
-                                                      Counters cnt) {
+__global__ __launch_bounds__(256) void k_walk_f64(const NodeRec64 *__restrict__ nodes,
+                                                  const double4 *__restrict__ pos,
+                                                  WalkArgs<F64> a) {
 #pragma clang fp contract(off)
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
-  if (wbase >= n) return;
+  if (wbase >= a.n) return;
   const uint32_t me = wbase + lane;
-  const bool valid = me < n;
+  const bool valid = me < a.n;
   const double4 p = valid ? pos[me] : make_double4(0.0, 0.0, 0.0, 0.0);
   double sx = 0.0, sy = 0.0, sz = 0.0;
   int D = valid ? 0x7fffffff : -1;
-  unsigned long long nt = 0, na = 0, np = 0;
+  unsigned long long nt = 0, na = 0, np = 0, wv = 0, wl = 0;
   uint32_t h = 1;
   int depth = 0;
   while (h != 0) {
     const NodeRec64 r = nodes[uni(h) - 1];
     const bool active = depth <= D;
-    if (r.count <= leaf) {
+    if (COUNT) ++wv;
+    if (r.count <= a.leaf) {
+      if (COUNT) ++wl;
       if (active) {
         for (uint32_t jj = 0; jj < r.count; ++jj) {
           const uint32_t j = r.start + jj;
@@ -107,7 +393,7 @@ __global__ __launch_bounds__(256) void k_traverse_f64(const NodeRec64 *__restric
       double dz = p.z - r.cz;
       double d2 = dx * dx + dy * dy + dz * dz;
       if (COUNT) ++nt;
-      if (r.size2 < theta2 * d2) {
+      if (r.size2 < a.theta2 * d2) {
         double dist = sqrt(d2);
         double magi = -r.m / (d2 * dist);
         sx += dx * magi;
@@ -128,210 +414,63 @@ __global__ __launch_bounds__(256) void k_traverse_f64(const NodeRec64 *__restric
     }
   }
   if (valid) {
-    ax[me] = sx;
-    ay[me] = sy;
-    az[me] = sz;
+    a.ax[me] = sx;
+    a.ay[me] = sy;
+    a.az[me] = sz;
   }
-  flush_counters<COUNT>(cnt, nt, na, np);
+  if (COUNT) {
+    nt = wave_sum_u64(nt);
+    na = wave_sum_u64(na);
+    np = wave_sum_u64(np);
+    if (lane == 0) {
+      atomicAdd(&a.cnt.c[0], nt);
+      atomicAdd(&a.cnt.c[1], na);
+      atomicAdd(&a.cnt.c[2], np);
+      atomicAdd(&a.cnt.c[3], wv);
+      atomicAdd(&a.cnt.c[4], wl);
+    }
+  }
 }
 
-// Compensated FP32 accumulator (Kahan): sum s, running compensation c.
-struct Kahan3 {
-  float s[3] = {0.f, 0.f, 0.f};
-  float c[3] = {0.f, 0.f, 0.f};
-  __device__ __forceinline__ void add(float tx, float ty, float tz) {
-    float t[3] = {tx, ty, tz};
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      float y = t[i] - c[i];
-      float u = s[i] + y;
-      c[i] = (u - s[i]) - y;
-      s[i] = u;
-    }
+template <class P>
+static void launch_policy(const WalkArgs<P> &a, bool count, int variant, hipStream_t s) {
+  const unsigned blocks = (a.n + 255) / 256;
+  const bool pred = (variant & 2) != 0;
+  if (count) {
+    if (pred)
+      k_walk<P, true, true><<<blocks, 256, 0, s>>>(a.nodes - 1, a.bodies, a);
+    else
+      k_walk<P, true, false><<<blocks, 256, 0, s>>>(a.nodes - 1, a.bodies, a);
+  } else {
+    if (pred)
+      k_walk<P, false, true><<<blocks, 256, 0, s>>>(a.nodes - 1, a.bodies, a);
+    else
+      k_walk<P, false, false><<<blocks, 256, 0, s>>>(a.nodes - 1, a.bodies, a);
   }
-  __device__ __forceinline__ double get(int i) const { return (double)s[i] - (double)c[i]; }
-};
-
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_traverse_mixed(const NodeRec32 *__restrict__ nodes,
-                                                        const float4 *__restrict__ pos,
-                                                        uint32_t n, float theta2,
-                                                        uint32_t leaf,
-                                                        double *__restrict__ ax,
-                                                        double *__restrict__ ay,
-                                                        double *__restrict__ az,
-                                                        Counters cnt) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
-  if (wbase >= n) return;
-  const uint32_t me = wbase + lane;
-  const bool valid = me < n;
-  const float4 p = valid ? pos[me] : make_float4(0.f, 0.f, 0.f, 0.f);
-  Kahan3 acc;
-  int D = valid ? 0x7fffffff : -1;
-  unsigned long long nt = 0, na = 0, np = 0;
-  uint32_t h = 1;
-  int depth = 0;
-  while (h != 0) {
-    const NodeRec32 r = nodes[uni(h) - 1];
-    const bool active = depth <= D;
-    if (r.count <= leaf) {
-      if (active) {
-        for (uint32_t jj = 0; jj < r.count; ++jj) {
-          const uint32_t j = r.start + jj;
-          const float4 q = pos[j];
-          if (j != me) {
-            float dx = p.x - q.x;
-            float dy = p.y - q.y;
-            float dz = p.z - q.z;
-            float d2 = dx * dx + dy * dy + dz * dz;
-            float ri = __builtin_amdgcn_rsqf(d2);
-            float f = -q.w * (ri * ri * ri);
-            acc.add(f * dx, f * dy, f * dz);
-            if (COUNT) ++np;
-          }
-        }
-      }
-      h = heap_next(h, depth);
-      continue;
-    }
-    bool open = false;
-    if (active) {
-      float dx = p.x - r.cx;
-      float dy = p.y - r.cy;
-      float dz = p.z - r.cz;
-      float d2 = dx * dx + dy * dy + dz * dz;
-      if (COUNT) ++nt;
-      if (r.size2 < theta2 * d2) {
-        float ri = __builtin_amdgcn_rsqf(d2);
-        float f = -r.m * (ri * ri * ri);
-        acc.add(f * dx, f * dy, f * dz);
-        D = depth;
-        if (COUNT) ++na;
-      } else {
-        D = 0x7fffffff;
-        open = true;
-      }
-    }
-    if (__any(open)) {
-      h = 2 * h;
-      ++depth;
-    } else {
-      h = heap_next(h, depth);
-    }
-  }
-  if (valid) {
-    ax[me] = acc.get(0);
-    ay[me] = acc.get(1);
-    az[me] = acc.get(2);
-  }
-  flush_counters<COUNT>(cnt, nt, na, np);
 }
 
-// Mixed precision with the coordinate difference taken in FP64.
-template <bool COUNT>
-__global__ __launch_bounds__(256) void k_traverse_mixed_d64(const NodeRecD *__restrict__ nodes,
-                                                            const double4 *__restrict__ pos,
-                                                            uint32_t n, float theta2,
-                                                            uint32_t leaf,
-                                                            double *__restrict__ ax,
-                                                            double *__restrict__ ay,
-                                                            double *__restrict__ az,
-                                                            Counters cnt) {
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
-  if (wbase >= n) return;
-  const uint32_t me = wbase + lane;
-  const bool valid = me < n;
-  const double4 p = valid ? pos[me] : make_double4(0.0, 0.0, 0.0, 0.0);
-  Kahan3 acc;
-  int D = valid ? 0x7fffffff : -1;
-  unsigned long long nt = 0, na = 0, np = 0;
-  uint32_t h = 1;
-  int depth = 0;
-  while (h != 0) {
-    const NodeRecD r = nodes[uni(h) - 1];
-    const bool active = depth <= D;
-    if (r.count <= leaf) {
-      if (active) {
-        for (uint32_t jj = 0; jj < r.count; ++jj) {
-          const uint32_t j = r.start + jj;
-          const double4 q = pos[j];
-          if (j != me) {
-            float dx = (float)(p.x - q.x);
-            float dy = (float)(p.y - q.y);
-            float dz = (float)(p.z - q.z);
-            float d2 = dx * dx + dy * dy + dz * dz;
-            float ri = __builtin_amdgcn_rsqf(d2);
-            float f = -(float)q.w * (ri * ri * ri);
-            acc.add(f * dx, f * dy, f * dz);
-            if (COUNT) ++np;
-          }
-        }
-      }
-      h = heap_next(h, depth);
-      continue;
-    }
-    bool open = false;
-    if (active) {
-      float dx = (float)(p.x - r.cx);
-      float dy = (float)(p.y - r.cy);
-      float dz = (float)(p.z - r.cz);
-      float d2 = dx * dx + dy * dy + dz * dz;
-      if (COUNT) ++nt;
-      if (r.size2 < theta2 * d2) {
-        float ri = __builtin_amdgcn_rsqf(d2);
-        float f = -r.m * (ri * ri * ri);
-        acc.add(f * dx, f * dy, f * dz);
-        D = depth;
-        if (COUNT) ++na;
-      } else {
-        D = 0x7fffffff;
-        open = true;
-      }
-    }
-    if (__any(open)) {
-      h = 2 * h;
-      ++depth;
-    } else {
-      h = heap_next(h, depth);
-    }
-  }
-  if (valid) {
-    ax[me] = acc.get(0);
-    ay[me] = acc.get(1);
-    az[me] = acc.get(2);
-  }
-  flush_counters<COUNT>(cnt, nt, na, np);
-}
-
-// ---------------------------------------------------------------------------
-// host launchers
-// ---------------------------------------------------------------------------
-
-hipError_t launch_traverse(int precision, bool count, const TreeDev &t,
+hipError_t launch_traverse(int precision, bool count, int variant, const TreeDev &t,
                            const double4 *pos64, const float4 *pos32, uint32_t n,
                            double theta, int leaf, double *ax, double *ay, double *az,
                            unsigned long long *counters, hipStream_t s) {
-  const unsigned blocks = (n + 255) / 256;
   Counters c{counters};
   const double theta2 = theta * theta;
-  const float theta2f = (float)theta2;
   if (precision == 0) {
-    if (count)
-      k_traverse_f64<true><<<blocks, 256, 0, s>>>(t.rec64, pos64, n, theta2, leaf, ax, ay, az, c);
-    else
-      k_traverse_f64<false><<<blocks, 256, 0, s>>>(t.rec64, pos64, n, theta2, leaf, ax, ay, az, c);
+    WalkArgs<F64> a{t.rec64, pos64, n, (uint32_t)leaf, theta2, ax, ay, az, c};
+    const unsigned blocks = (n + 255) / 256;
+    if (variant & 4) {
+      launch_policy(a, count, variant, s);
+    } else if (count) {
+      k_walk_f64<true><<<blocks, 256, 0, s>>>(a.nodes, a.bodies, a);
+    } else {
+      k_walk_f64<false><<<blocks, 256, 0, s>>>(a.nodes, a.bodies, a);
+    }
   } else if (precision == 1) {
-    if (count)
-      k_traverse_mixed<true><<<blocks, 256, 0, s>>>(t.rec32, pos32, n, theta2f, leaf, ax, ay, az, c);
-    else
-      k_traverse_mixed<false><<<blocks, 256, 0, s>>>(t.rec32, pos32, n, theta2f, leaf, ax, ay, az, c);
+    WalkArgs<F32> a{t.rec32, pos32, n, (uint32_t)leaf, theta2, ax, ay, az, c};
+    launch_policy(a, count, variant, s);
   } else {
-    if (count)
-      k_traverse_mixed_d64<true><<<blocks, 256, 0, s>>>(t.recd, pos64, n, theta2f, leaf, ax, ay, az, c);
-    else
-      k_traverse_mixed_d64<false><<<blocks, 256, 0, s>>>(t.recd, pos64, n, theta2f, leaf, ax, ay, az, c);
+    WalkArgs<D64> a{t.recd, pos64, n, (uint32_t)leaf, theta2, ax, ay, az, c};
+    launch_policy(a, count, variant, s);
   }
   return hipGetLastError();
 }

@@ -123,9 +123,14 @@ class Simulation:
         check(self._L.kd_sync(self._ctx))
 
     def counters(self) -> np.ndarray:
-        out = np.zeros(3, dtype=np.int64)
+        """[node tests, M2P, P2P] summed over bodies, then [node visits, leaf
+        visits] summed over wavefronts."""
+        out = np.zeros(5, dtype=np.int64)
         check(self._L.kd_counters(self._ctx, iptr(out)))
         return out
+
+    def set_tuning(self, key: str, value: int):
+        check(self._L.kd_set_tuning(self._ctx, key.encode(), int(value)))
 
     def timings(self) -> dict:
         """Summed HIP-event times (ms) of kd_step phases since the last reset."""
