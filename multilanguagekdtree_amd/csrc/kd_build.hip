@@ -533,7 +533,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   }
   __syncthreads();
 
-  uint32_t st[IT], ct[IT], cc[IT];
+  uint32_t st[IT], ct[IT], cc[IT], hl[IT];
   bool alive[IT];
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
@@ -542,47 +542,33 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
     st[it] = 0;
     ct[it] = n0;
     cc[it] = 0;
+    hl[it] = h0;
   }
   int cur = 0;
   int lam_end = 0;
   for (int lam = 0;; ++lam) {
     const uint32_t hbase = (1u << (a.level + lam)) - 1 + (k << lam);
-    // segment owners: leaf emission or split decision
-#pragma unroll
-    for (int it = 0; it < IT; ++it) {
-      uint32_t j = tid * IT + it;
-      if (!alive[it] || j != st[it]) continue;
-      uint32_t h = hbase + cc[it];
-      uint32_t sgs = st[it], sgc = ct[it];
+    // one thread per segment of this level: split decision or leaf mark
+    for (uint32_t c = tid; c < (1u << lam); c += NT) {
+      uint32_t sgs = 0, sgc = n0;
+      bool ok = true;
+      for (int bb = lam - 1; bb >= 0; --bb) {
+        if (sgc <= (uint32_t)leaf) {
+          ok = false;  // an ancestor is a leaf: no such node
+          break;
+        }
+        const uint32_t half = sgc >> 1;
+        if ((c >> bb) & 1u) {
+          sgs += half;
+          sgc -= half;
+        } else {
+          sgc = half;
+        }
+      }
+      if (!ok) continue;
+      const uint32_t h = hbase + c;
       if (sgc <= (uint32_t)leaf) {
-        // canonical member order: ascending original index (insertion sort
-        // of the leaf's slice of list 0, at most MAX_LEAF entries)
-        uint16_t *l0 = &lst[cur][0][sgs];
-        for (uint32_t x = 1; x < sgc; ++x) {
-          uint16_t v = l0[x];
-          uint32_t ov = a.in.oid[ids[v]];
-          int y = (int)x - 1;
-          while (y >= 0 && a.in.oid[ids[l0[y]]] > ov) {
-            l0[y + 1] = l0[y];
-            --y;
-          }
-          l0[y + 1] = v;
-        }
-        double mm = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
-        for (uint32_t x = 0; x < sgc; ++x) {
-          uint32_t id = ids[l0[x]];
-          double mi = a.m[id];
-          mm += mi;
-          mx += mi * a.P.c[0][id];
-          my += mi * a.P.c[1][id];
-          mz += mi * a.P.c[2][id];
-        }
-        a.t.mm[h] = mm;
-        a.t.mx[h] = mx;
-        a.t.my[h] = my;
-        a.t.mz[h] = mz;
-        write_leaf_rec(a.t, h, s0 + sgs, sgc);
-        sdarr[sgs] = 0xFF;
+        sdarr[sgs] = 0xFF;  // leaf: its bodies are finished at this level
       } else {
         double ext[3];
 #pragma unroll
@@ -624,7 +610,10 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
     int any = 0;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
-      if (alive[it] && sdarr[st[it]] == 0xFF) alive[it] = false;
+      if (alive[it] && sdarr[st[it]] == 0xFF) {
+        alive[it] = false;
+        hl[it] = hbase + cc[it];  // heap slot of the leaf this body ends in
+      }
       any |= alive[it] ? 1 : 0;
     }
     lam_end = lam;
@@ -704,10 +693,35 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
     }
   }
 
-  // tree-order body state; list 0 now holds every leaf's members in order
+  // Leaves in canonical order.  List 0 holds every leaf's members (in list
+  // order); each body ranks itself among its leaf by original index, using
+  // the spare list buffer as staging for those indices, and its state goes
+  // straight to its slot in tree order.
+  uint32_t *oidl = reinterpret_cast<uint32_t *>(&lst[cur ^ 1][0][0]);  // 16 KiB
+  uint16_t *permloc = &lst[cur ^ 1][2][0];  // tree position -> local index
+  uint32_t ow[IT];
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    uint32_t j = tid * IT + it;
+    ow[it] = 0;
+    if (j < n0) {
+      ow[it] = a.in.oid[ids[lst[cur][0][j]]];
+      oidl[j] = ow[it];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    uint32_t j = tid * IT + it;
+    if (j >= n0) continue;
+    uint32_t rank = 0;
+    for (uint32_t q = st[it]; q < st[it] + ct[it]; ++q) rank += oidl[q] < ow[it] ? 1u : 0u;
+    permloc[st[it] + rank] = lst[cur][0][j];
+  }
+  __syncthreads();
   for (uint32_t i = tid; i < n0; i += NT) {
-    uint32_t w = ids[lst[cur][0][i]];
-    uint32_t tpos = s0 + i;
+    const uint32_t w = ids[permloc[i]];
+    const uint32_t tpos = s0 + i;
     a.perm[tpos] = w;
     double x = a.in.x[w], y = a.in.y[w], z = a.in.z[w], m = a.in.m[w];
     a.out.x[tpos] = x;
@@ -720,6 +734,28 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
     a.out.oid[tpos] = a.in.oid[w];
     a.pos64[tpos] = make_double4(x, y, z, m);
     a.pos32[tpos] = make_float4((float)x, (float)y, (float)z, (float)m);
+  }
+  __threadfence_block();
+  __syncthreads();
+  // leaf moments, summed in canonical order from the tree-ordered records
+#pragma unroll
+  for (int it = 0; it < IT; ++it) {
+    uint32_t j = tid * IT + it;
+    if (j >= n0 || j != st[it]) continue;
+    double mm = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    for (uint32_t q = 0; q < ct[it]; ++q) {
+      const double4 b = a.pos64[s0 + st[it] + q];
+      mm += b.w;
+      mx += b.w * b.x;
+      my += b.w * b.y;
+      mz += b.w * b.z;
+    }
+    const uint32_t h = hl[it];
+    a.t.mm[h] = mm;
+    a.t.mx[h] = mx;
+    a.t.my[h] = my;
+    a.t.mz[h] = mz;
+    write_leaf_rec(a.t, h, s0 + st[it], ct[it]);
   }
 
   // moments bottom-up over this subtree's internal nodes
