@@ -121,6 +121,17 @@ int kd_export_tree(kd_ctx *ctx, int32_t layout, kd_tree_view *view);
 /* Node-vector length the reference would allocate for n bodies. */
 int64_t kd_nodes_needed(int64_t n, int32_t leaf_size, int32_t layout);
 
+/* calc_accel(p, particles, nodes) over a caller-supplied node vector in one
+ * of the reference layouts (array_kd_tree.rs:619 / Sequential :174): the
+ * tree is not rebuilt, it is traversed as given (FP64 reference arithmetic).
+ * Leaf members are tree->perm[first .. first+num_parts) (body indices).
+ * Computes the k bodies idx[0..k) (idx NULL: all n bodies in order).
+ * Stateless: host arrays in and out, on the current HIP device.  A tree
+ * whose indices leave their arrays is rejected with KD_ERR_ARG. */
+int kd_accel_tree(int64_t n, const double *x, const double *y, const double *z,
+                  const double *m, const kd_tree_view *tree, double theta, int64_t k,
+                  const int64_t *idx, double *ax, double *ay, double *az);
+
 /* Build, then accelerations of every body: calc_accel(p, ..) for all p
  * (array_kd_tree.rs:619 / Sequential :174), in original order. */
 int kd_accel(kd_ctx *ctx, double *ax, double *ay, double *az);

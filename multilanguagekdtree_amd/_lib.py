@@ -82,6 +82,8 @@ SIGNATURES = {
     "kd_build": (C.c_int, [C.c_void_p]),
     "kd_export_tree": (C.c_int, [C.c_void_p, C.c_int32, _P(KdTreeView)]),
     "kd_nodes_needed": (C.c_int64, [C.c_int64, C.c_int32, C.c_int32]),
+    "kd_accel_tree": (C.c_int, [C.c_int64] + [_d] * 4 + [_P(KdTreeView), C.c_double,
+                                                          C.c_int64, _i64, _d, _d, _d]),
     "kd_accel": (C.c_int, [C.c_void_p, _d, _d, _d]),
     "kd_accel_device": (C.c_int, [C.c_void_p] + [C.c_void_p] * 3),
     "kd_step": (C.c_int, [C.c_void_p, C.c_int64]),

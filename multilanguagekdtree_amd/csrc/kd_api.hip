@@ -768,3 +768,104 @@ int kd_ic_two_bodies(double *x, double *y, double *z, double *vx, double *vy, do
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// traversal of a caller-supplied node vector
+// ---------------------------------------------------------------------------
+
+extern "C" int kd_accel_tree(int64_t n, const double *x, const double *y, const double *z,
+                             const double *m, const kd_tree_view *tv, double theta, int64_t k,
+                             const int64_t *idx, double *ax, double *ay, double *az) {
+  if (n < 1 || !x || !y || !z || !m || !tv || !ax || !ay || !az)
+    return fail(KD_ERR_ARG, "bad argument to kd_accel_tree");
+  if (!idx) k = n;
+  if (k < 1) return fail(KD_ERR_ARG, "k must be >= 1");
+  if (!(theta >= 0.0) || !std::isfinite(theta)) return fail(KD_ERR_ARG, "bad theta");
+  const int64_t nn = tv->n_nodes;
+  if (nn < 1 || !tv->is_leaf || !tv->num_parts || !tv->first || !tv->m || !tv->cmx ||
+      !tv->cmy || !tv->cmz || !tv->size || !tv->left || !tv->right || !tv->perm)
+    return fail(KD_ERR_ARG, "incomplete tree view");
+  // every index the kernel can follow must stay inside its array
+  for (int64_t i = 0; i < n; ++i)
+    if (tv->perm[i] < 0 || tv->perm[i] >= n)
+      return fail(KD_ERR_ARG, "perm[%lld] out of range", (long long)i);
+  for (int64_t i = 0; i < nn; ++i) {
+    if (tv->is_leaf[i]) {
+      if (tv->num_parts[i] < 0 || tv->first[i] < 0 || tv->first[i] + tv->num_parts[i] > n)
+        return fail(KD_ERR_ARG, "leaf %lld members out of range", (long long)i);
+    } else if (tv->left[i] < 0 || tv->left[i] >= nn || tv->right[i] < 0 || tv->right[i] >= nn) {
+      return fail(KD_ERR_ARG, "node %lld children out of range", (long long)i);
+    }
+  }
+  {
+    // the walk visits every node it reaches at most once per wavefront only if
+    // the node vector is a tree: reject shared subtrees and cycles, and trees
+    // deeper than the per-wavefront stack
+    std::vector<uint8_t> seen((size_t)nn, 0);
+    std::vector<std::pair<int64_t, int>> todo{{0, 0}};
+    while (!todo.empty()) {
+      const auto [h, d] = todo.back();
+      todo.pop_back();
+      if (seen[h]) return fail(KD_ERR_ARG, "node %lld reached twice: not a tree", (long long)h);
+      seen[h] = 1;
+      if (d >= PW_STACK) return fail(KD_ERR_ARG, "tree deeper than %d levels", PW_STACK);
+      if (!tv->is_leaf[h]) {
+        todo.push_back({tv->right[h], d + 1});
+        todo.push_back({tv->left[h], d + 1});
+      }
+    }
+  }
+  if (idx)
+    for (int64_t i = 0; i < k; ++i)
+      if (idx[i] < 0 || idx[i] >= n) return fail(KD_ERR_ARG, "idx[%lld] out of range", (long long)i);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(KD_ERR_NODEVICE, "no HIP device visible");
+
+  std::vector<void *> bufs;
+  auto release = [&]() {
+    for (void *p : bufs) (void)hipFree(p);
+  };
+  auto dcopy = [&](const void *src, size_t bytes, void **dst) -> hipError_t {
+    hipError_t e = hipMalloc(dst, bytes ? bytes : 8);
+    if (e != hipSuccess) return e;
+    bufs.push_back(*dst);
+    return src ? hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  hipError_t e = hipSuccess;
+  void *dx, *dy, *dz, *dm, *dil, *dnp, *dfi, *dnm, *dcx, *dcy, *dcz, *dsz, *dl, *dr, *dp, *didx = nullptr;
+  void *oa, *ob, *oc, *derr;
+  const size_t N8 = sizeof(double) * n, M8 = sizeof(double) * nn, MI = sizeof(int64_t) * nn;
+  if ((e = dcopy(x, N8, &dx)) || (e = dcopy(y, N8, &dy)) || (e = dcopy(z, N8, &dz)) ||
+      (e = dcopy(m, N8, &dm)) || (e = dcopy(tv->is_leaf, sizeof(int32_t) * nn, &dil)) ||
+      (e = dcopy(tv->num_parts, MI, &dnp)) || (e = dcopy(tv->first, MI, &dfi)) ||
+      (e = dcopy(tv->m, M8, &dnm)) || (e = dcopy(tv->cmx, M8, &dcx)) ||
+      (e = dcopy(tv->cmy, M8, &dcy)) || (e = dcopy(tv->cmz, M8, &dcz)) ||
+      (e = dcopy(tv->size, M8, &dsz)) || (e = dcopy(tv->left, MI, &dl)) ||
+      (e = dcopy(tv->right, MI, &dr)) || (e = dcopy(tv->perm, sizeof(int64_t) * n, &dp)) ||
+      (idx && (e = dcopy(idx, sizeof(int64_t) * k, &didx))) ||
+      (e = dcopy(nullptr, sizeof(double) * k, &oa)) || (e = dcopy(nullptr, sizeof(double) * k, &ob)) ||
+      (e = dcopy(nullptr, sizeof(double) * k, &oc)) || (e = dcopy(nullptr, sizeof(int), &derr))) {
+    release();
+    return fail(e == hipErrorOutOfMemory ? KD_ERR_NOMEM : KD_ERR_HIP, "kd_accel_tree: %s",
+                hipGetErrorString(e));
+  }
+  int herr = 0;
+  PtrTree t{(const int32_t *)dil, (const int64_t *)dnp, (const int64_t *)dfi,
+            (const double *)dnm, (const double *)dcx, (const double *)dcy,
+            (const double *)dcz, (const double *)dsz, (const int64_t *)dl,
+            (const int64_t *)dr, (const int64_t *)dp, nn};
+  e = hipMemset(derr, 0, sizeof(int));
+  if (e == hipSuccess)
+    e = launch_walk_ptr(t, (const double *)dx, (const double *)dy, (const double *)dz,
+                        (const double *)dm, n, (const int64_t *)didx, k, theta, (double *)oa,
+                        (double *)ob, (double *)oc, (int *)derr, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(ax, oa, sizeof(double) * k, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(ay, ob, sizeof(double) * k, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(az, oc, sizeof(double) * k, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&herr, derr, sizeof(int), hipMemcpyDeviceToHost);
+  release();
+  if (e != hipSuccess) return fail(KD_ERR_HIP, "kd_accel_tree: %s", hipGetErrorString(e));
+  if (herr) return fail(KD_ERR_ARG, "tree deeper than the traversal stack allows");
+  return KD_OK;
+}

@@ -118,4 +118,18 @@ hipError_t launch_sanitize(const StateDev &s, uint32_t n, int *bad, hipStream_t 
 hipError_t launch_direct(const StateDev &s, uint32_t n, const uint32_t *wpos, uint32_t k,
                          double *out, hipStream_t st);
 
+constexpr int PW_STACK = 96;  // traversal stack entries per wavefront (bounds tree depth)
+struct PtrTree {
+  const int32_t *is_leaf;
+  const int64_t *num_parts, *first;
+  const double *m, *cmx, *cmy, *cmz, *size;
+  const int64_t *left, *right;
+  const int64_t *parts;  // leaf members: parts[first .. first+num_parts)
+  int64_t n_nodes;
+};
+hipError_t launch_walk_ptr(const PtrTree &t, const double *x, const double *y, const double *z,
+                           const double *m, int64_t n, const int64_t *idx, int64_t k,
+                           double theta, double *ax, double *ay, double *az, int *err,
+                           hipStream_t s);
+
 }  // namespace kd

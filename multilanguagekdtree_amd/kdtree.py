@@ -39,6 +39,33 @@ class Tree:
         f = self.first[node]
         return self.perm[f:f + self.num_parts[node]]
 
+    def view(self) -> tuple[KdTreeView, list]:
+        """A kd_tree_view over these arrays (and the arrays it points into,
+        which must stay alive while the view is used)."""
+        keep = dict(
+            is_leaf=np.ascontiguousarray(self.is_leaf, np.int32),
+            split_dim=np.ascontiguousarray(self.split_dim, np.int32),
+            num_parts=np.ascontiguousarray(self.num_parts, np.int64),
+            first=np.ascontiguousarray(self.first, np.int64),
+            split_val=np.ascontiguousarray(self.split_val, np.float64),
+            m=np.ascontiguousarray(self.m, np.float64),
+            cmx=np.ascontiguousarray(self.cm[:, 0], np.float64),
+            cmy=np.ascontiguousarray(self.cm[:, 1], np.float64),
+            cmz=np.ascontiguousarray(self.cm[:, 2], np.float64),
+            size=np.ascontiguousarray(self.size, np.float64),
+            left=np.ascontiguousarray(self.left, np.int64),
+            right=np.ascontiguousarray(self.right, np.int64),
+            perm=np.ascontiguousarray(self.perm, np.int64))
+        v = KdTreeView()
+        v.n_nodes = len(keep["is_leaf"])
+        v.is_leaf = i32ptr(keep["is_leaf"])
+        v.split_dim = i32ptr(keep["split_dim"])
+        for k in ("num_parts", "first", "left", "right", "perm"):
+            setattr(v, k, iptr(keep[k]))
+        for k in ("split_val", "m", "cmx", "cmy", "cmz", "size"):
+            setattr(v, k, dptr(keep[k]))
+        return v, list(keep.values())
+
 
 class Simulation:
     """One device context (kd_ctx) holding up to ``capacity`` bodies."""
@@ -215,6 +242,26 @@ def calc_accel(bodies: dict, theta: float = 0.3, leaf_size: int = 7,
         return s.accel()
 
 
+def calc_accel_tree(bodies: dict, tree: Tree, theta: float = 0.3,
+                    idx: np.ndarray | None = None) -> np.ndarray:
+    """calc_accel(p, particles, nodes) over a given node vector
+    (array_kd_tree.rs:619 / Sequential :174) for the bodies ``idx`` (default:
+    all), FP64; the tree is traversed as given, not rebuilt.  Shape (k, 3)."""
+    n = len(bodies["x"])
+    cols = [np.ascontiguousarray(bodies[f], np.float64) for f in ("x", "y", "z", "m")]
+    v, keep = tree.view()
+    if idx is None:
+        k, ip = n, None
+    else:
+        idx = np.ascontiguousarray(idx, np.int64)
+        k, ip = len(idx), iptr(idx)
+    out = [np.zeros(k) for _ in range(3)]
+    check(_lib.lib().kd_accel_tree(n, *[dptr(c) for c in cols], C.byref(v), theta, k, ip,
+                                   *[dptr(o) for o in out]))
+    del keep
+    return np.stack(out, axis=1)
+
+
 def circular_orbits(n: int, seed: int = 0) -> dict:
     """circular_orbits(n) of array_particle.rs:19-44 (n+1 bodies), seeded."""
     return _lib.ic("circular_orbits", n, seed)
@@ -231,6 +278,6 @@ def ring(n: int, seed: int = 0) -> dict:
 
 
 __all__ = ["Simulation", "Tree", "allocate_node_vec", "simple_sim", "build_tree",
-           "calc_accel", "circular_orbits", "two_bodies", "ring",
+           "calc_accel", "calc_accel_tree", "circular_orbits", "two_bodies", "ring",
            "KD_PREC_FP64", "KD_PREC_MIXED", "KD_PREC_MIXED_D64",
            "KD_LAYOUT_COMPACT", "KD_LAYOUT_PADDED"]

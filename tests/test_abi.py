@@ -91,3 +91,43 @@ def test_bad_arguments_rejected_on_host():
     assert L.kd_create(10, C.byref(p), C.byref(ctx)) == _lib.KD_ERR_ARG
     assert b"leaf_size" in L.kd_last_error()
     assert L.kd_step(None, 1) == _lib.KD_ERR_ARG
+
+
+def _chain(n):
+    nn = 2 * n - 1
+    t = K.Tree(is_leaf=np.zeros(nn, np.int32), split_dim=np.zeros(nn, np.int32),
+               num_parts=np.zeros(nn, np.int64), first=np.zeros(nn, np.int64),
+               split_val=np.zeros(nn), m=np.zeros(nn), cm=np.zeros((nn, 3)),
+               size=np.ones(nn), left=np.zeros(nn, np.int64), right=np.zeros(nn, np.int64),
+               perm=np.arange(n, dtype=np.int64))
+    for j in range(n - 1):
+        t.left[2 * j], t.right[2 * j] = 2 * j + 1, 2 * j + 2
+        t.is_leaf[2 * j + 1], t.num_parts[2 * j + 1], t.first[2 * j + 1] = 1, 1, j
+    t.is_leaf[nn - 1], t.num_parts[nn - 1], t.first[nn - 1] = 1, 1, n - 1
+    return t
+
+
+@pytest.mark.parametrize("defect", ["child", "leaf_range", "perm", "cycle", "shared",
+                                    "deep", "idx"])
+def test_malformed_tree_rejected_on_host(defect):
+    n = 8
+    b = K.ring(n, seed=1)
+    t = _chain(n if defect != "deep" else 120)
+    if defect == "deep":
+        b = K.ring(120, seed=1)
+    idx = None
+    if defect == "child":
+        t.right[2] = len(t.is_leaf)
+    elif defect == "leaf_range":
+        t.first[3] = n - 0  # one member past the end
+    elif defect == "perm":
+        t.perm[4] = -1
+    elif defect == "cycle":
+        t.right[4] = 0
+    elif defect == "shared":
+        t.right[4] = 3  # a leaf reached from two parents
+    elif defect == "idx":
+        idx = np.array([0, n], np.int64)
+    with pytest.raises(_lib.KdError) as e:
+        K.calc_accel_tree(b, t, theta=0.0, idx=idx)
+    assert e.value.code == _lib.KD_ERR_ARG
