@@ -171,6 +171,45 @@ int kd_direct_sum(kd_ctx *ctx, int64_t k, const int64_t *idx, double *ax,
                   double *ay, double *az, double *absx, double *absy,
                   double *absz);
 
+/* ---- batches: many independent systems in one context ----
+ * A batch holds nsys independent systems of n[i] bodies each and runs
+ * simple_sim (Parallel/RustVersion/src/array_kd_tree.rs:623-664) on all of
+ * them at once: one build of the whole forest (every tree with the
+ * reference's split rule and leaf size), one walk over all systems, one
+ * kick-drift.  Bodies are passed concatenated in system order; results come
+ * back the same way.  kd_batch_step replays the step as a HIP graph (two
+ * graphs, one per state buffer), captured on first use.  No counterpart in
+ * the reference beyond the per-system calls it replaces. */
+typedef struct kd_batch kd_batch;
+int kd_batch_create(int64_t nsys, const int64_t *n, const kd_params *params,
+                    kd_batch **out);
+int kd_batch_destroy(kd_batch *b);
+/* Bodies over all systems (sum of n[i]). */
+int64_t kd_batch_size(const kd_batch *b);
+int kd_batch_upload(kd_batch *b, const double *x, const double *y, const double *z,
+                    const double *vx, const double *vy, const double *vz,
+                    const double *m);
+int kd_batch_download(kd_batch *b, double *x, double *y, double *z, double *vx,
+                      double *vy, double *vz);
+/* Build every tree, then calc_accel of every body (array_kd_tree.rs:619). */
+int kd_batch_accel(kd_batch *b, double *ax, double *ay, double *az);
+/* steps x (build, accel, kick-drift) on every system. */
+int kd_batch_step(kd_batch *b, int64_t steps);
+/* Tree of system sys in a reference layout, as kd_export_tree. */
+int kd_batch_export_tree(kd_batch *b, int64_t sys, int32_t layout, kd_tree_view *view);
+/* Launch-shape knobs: "graph" 0/1 (replay the step as a HIP graph),
+ * "walk_threads" 64/128/256/512 (threads per block of the walk),
+ * "walk_waves_per_cu" 0..64 (0: one wavefront per task; otherwise a
+ * persistent grid of that many wavefronts per CU takes tasks from a queue). */
+int kd_batch_set_tuning(kd_batch *b, const char *key, int64_t value);
+/* Time the walk over the uploaded batch for every launch shape (best of reps
+ * runs each) and keep the fastest: out = {threads, waves_per_cu, ms}.  The
+ * walk is stackless, so its block size and persistence are the shape. */
+int kd_batch_autotune(kd_batch *b, int64_t reps, double out[3]);
+/* Counters as kd_counters, summed over the systems and over the traversals
+ * since the last kd_batch_accel / kd_batch_step began. */
+int kd_batch_counters(kd_batch *b, int64_t out[5]);
+
 /* Synthetic initial conditions (host only, no device needed).
  * kd_ic_ring: body 0 mass 1 at rest at the origin; bodies 1..n-1 mass 1e-14,
  * r ~ U[1,2), phi ~ U[0,2pi), z ~ N(0, 1e-3), v = sqrt(1/r)(-sin phi, cos phi, 0).

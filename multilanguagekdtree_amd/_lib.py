@@ -93,6 +93,17 @@ SIGNATURES = {
     "kd_timings": (C.c_int, [C.c_void_p, _d]),
     "kd_reset_timings": (C.c_int, [C.c_void_p]),
     "kd_direct_sum": (C.c_int, [C.c_void_p, C.c_int64, _i64] + [_d] * 6),
+    "kd_batch_create": (C.c_int, [C.c_int64, _i64, _P(KdParams), _P(C.c_void_p)]),
+    "kd_batch_destroy": (C.c_int, [C.c_void_p]),
+    "kd_batch_size": (C.c_int64, [C.c_void_p]),
+    "kd_batch_upload": (C.c_int, [C.c_void_p] + [_d] * 7),
+    "kd_batch_download": (C.c_int, [C.c_void_p] + [_d] * 6),
+    "kd_batch_accel": (C.c_int, [C.c_void_p, _d, _d, _d]),
+    "kd_batch_step": (C.c_int, [C.c_void_p, C.c_int64]),
+    "kd_batch_export_tree": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _P(KdTreeView)]),
+    "kd_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "kd_batch_autotune": (C.c_int, [C.c_void_p, C.c_int64, _d]),
+    "kd_batch_counters": (C.c_int, [C.c_void_p, _i64]),
     "kd_ic_ring": (C.c_int, [C.c_uint64, C.c_int64] + [_d] * 7),
     "kd_ic_circular_orbits": (C.c_int, [C.c_uint64, C.c_int64] + [_d] * 7),
     "kd_ic_two_bodies": (C.c_int, [_d] * 7),

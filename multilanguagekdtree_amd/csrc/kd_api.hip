@@ -72,6 +72,11 @@ struct kd_ctx {
   double t_ms[3] = {0.0, 0.0, 0.0};
   int64_t t_steps = 0;
   int traverse_variant = 0;  // bit 1: predicated leaf batches; bit 2: FP64 on the batched walk
+  // systems: one for a plain context, many for a batch
+  Forest forest;
+  std::vector<SysDesc> hsys;
+  SysDesc *dsys = nullptr;
+  uint32_t *seg_lo = nullptr, *seg_hi = nullptr;
 };
 
 template <typename T>
@@ -105,14 +110,11 @@ static int check_params(const kd_params *p) {
   return KD_OK;
 }
 
-static int ensure_nodes(kd_ctx *c) {
-  Shape sh = make_shape(c->n, c->params.leaf_size);
-  c->shape = sh;
-  if (sh.slots > c->slots_alloc) {
+static int alloc_nodes(kd_ctx *c, int64_t s) {
+  if (s > c->slots_alloc) {
     for (void *p : c->node_allocs) (void)hipFree(p);
     c->node_allocs.clear();
     c->slots_alloc = 0;
-    int64_t s = sh.slots;
     TreeDev &t = c->tree;
     ALLOC(c->node_allocs, t.seg_start, s);
     ALLOC(c->node_allocs, t.seg_count, s);
@@ -128,8 +130,44 @@ static int ensure_nodes(kd_ctx *c) {
     ALLOC(c->node_allocs, t.recd, s);
     c->slots_alloc = s;
   }
-  HIP_TRY(launch_shape(c->tree, sh, c->stream));
   return KD_OK;
+}
+
+// Copy the system table to the device and compute every tree's segments.
+static int install_forest(kd_ctx *c) {
+  const int nsys = (int)c->hsys.size();
+  Forest &f = c->forest;
+  f = Forest();
+  f.nsys = nsys;
+  f.sys = c->dsys;
+  f.seg_lo = c->seg_lo;
+  f.seg_hi = c->seg_hi;
+  std::vector<uint32_t> lo(nsys), hi(nsys);
+  for (int i = 0; i < nsys; ++i) {
+    const SysDesc &d = c->hsys[i];
+    f.max_sub_level = std::max(f.max_sub_level, (int)d.sub_level);
+    f.max_n = std::max(f.max_n, d.n);
+    f.max_slots = std::max(f.max_slots, d.slots);
+    f.total = std::max(f.total, d.body_off + d.n);
+    lo[i] = d.body_off;
+    hi[i] = d.body_off + d.n;
+  }
+  HIP_TRY(hipMemcpyAsync(c->dsys, c->hsys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice,
+                         c->stream));
+  HIP_TRY(hipMemcpyAsync(c->seg_lo, lo.data(), 4 * nsys, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->seg_hi, hi.data(), 4 * nsys, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(launch_shape(c->tree, f, c->params.leaf_size, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));  // the host tables above are temporaries
+  return KD_OK;
+}
+
+static int ensure_nodes(kd_ctx *c) {
+  Shape sh = make_shape(c->n, c->params.leaf_size);
+  c->shape = sh;
+  int rc = alloc_nodes(c, sh.slots);
+  if (rc) return rc;
+  c->hsys.assign(1, make_sys(sh, 0, 0));
+  return install_forest(c);
 }
 
 extern "C" {
@@ -174,7 +212,16 @@ int kd_destroy(kd_ctx *c) {
   return KD_OK;
 }
 
-static int create_impl(kd_ctx *c, int64_t cap) {
+// Buffer sizes beyond the body capacity: sort scratch, per-tile level
+// counts, tie flags, and the number of systems.
+struct Sizing {
+  size_t sort_bytes;
+  size_t cnt_entries;
+  size_t tieflag_entries;
+  int nsys;
+};
+
+static int create_impl(kd_ctx *c, int64_t cap, const Sizing &z) {
   HIP_TRY(hipGetDevice(&c->device));
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, c->device));
@@ -211,11 +258,13 @@ static int create_impl(kd_ctx *c, int64_t cap) {
   ALLOC(c->allocs, c->pos32, N + BODY_PAD);
   HIP_TRY(hipMemsetAsync(c->pos64, 0, sizeof(double4) * (N + BODY_PAD), c->stream));
   HIP_TRY(hipMemsetAsync(c->pos32, 0, sizeof(float4) * (N + BODY_PAD), c->stream));
-  size_t max_seg = 2 * (N / SUB_MAX) + 64;
-  ALLOC(c->allocs, c->b.cnt, 3 * (N / LVL_TILE + max_seg + 64));
-  ALLOC(c->allocs, c->b.tieflag, max_seg);
-  c->b.sort_tmp_bytes = sort_temp_bytes((uint32_t)cap);
+  ALLOC(c->allocs, c->b.cnt, z.cnt_entries);
+  ALLOC(c->allocs, c->b.tieflag, z.tieflag_entries);
+  c->b.sort_tmp_bytes = z.sort_bytes;
   ALLOC(c->allocs, *(uint8_t **)&c->b.sort_tmp, c->b.sort_tmp_bytes);
+  ALLOC(c->allocs, c->dsys, z.nsys);
+  ALLOC(c->allocs, c->seg_lo, z.nsys);
+  ALLOC(c->allocs, c->seg_hi, z.nsys);
   ALLOC(c->allocs, c->counters, 8);
   ALLOC(c->allocs, c->bad, 4);
   return KD_OK;
@@ -240,7 +289,13 @@ int kd_create(int64_t cap_n, const kd_params *params, kd_ctx **out) {
   kd_ctx *c = new kd_ctx();
   c->params = p;
   c->cap = cap_n;
-  rc = create_impl(c, cap_n);
+  // a single system of up to cap_n bodies: at most 2*cap/SUB_MAX segments
+  // reach the subtree builder, each device-wide level has at most
+  // cap/LVL_TILE + 2^level tiles
+  const size_t max_seg = 2 * ((size_t)cap_n / SUB_MAX) + 64;
+  Sizing z{sort_temp_bytes((uint32_t)cap_n, 1),
+           3 * ((size_t)cap_n / LVL_TILE + max_seg + 64), max_seg, 1};
+  rc = create_impl(c, cap_n, z);
   if (rc) {
     std::string msg = g_err;
     kd_destroy(c);
@@ -351,8 +406,8 @@ int kd_download_device(kd_ctx *c, double *x, double *y, double *z, double *vx, d
 
 static int build_impl(kd_ctx *c) {
   int nxt = c->cur ^ 1;
-  HIP_TRY(launch_build(c->shape, c->st[c->cur], c->st[nxt], c->tree, c->b, c->pos64, c->pos32,
-                       c->params.theta, c->stream));
+  HIP_TRY(launch_build(c->forest, c->params.leaf_size, c->st[c->cur], c->st[nxt], c->tree, c->b,
+                       c->pos64, c->pos32, c->params.theta, c->stream));
   c->cur = nxt;
   c->has_tree = true;
   return KD_OK;
@@ -631,13 +686,14 @@ static int64_t compact_used(int64_t n, int leaf) {
   return 1 + compact_used(n / 2, leaf) + compact_used(n - n / 2, leaf);
 }
 
-int kd_export_tree(kd_ctx *c, int32_t layout, kd_tree_view *v) {
-  if (!c || !v) return fail(KD_ERR_ARG, "NULL argument");
+// Export the tree of system d (heap at d.node_off, bodies at d.body_off).
+static int export_impl(kd_ctx *c, const SysDesc &d, int32_t layout, kd_tree_view *v) {
+  if (!v) return fail(KD_ERR_ARG, "NULL argument");
   if (!c->has_tree) return fail(KD_ERR_STATE, "no tree built");
   if (layout != KD_LAYOUT_COMPACT && layout != KD_LAYOUT_PADDED)
     return fail(KD_ERR_ARG, "unknown layout %d", layout);
   const int leaf = c->params.leaf_size;
-  const int64_t n = c->n;
+  const int64_t n = d.n;
   int64_t need;
   if (layout == KD_LAYOUT_PADDED) {
     need = kd_nodes_needed(n, leaf, layout);
@@ -652,7 +708,8 @@ int kd_export_tree(kd_ctx *c, int32_t layout, kd_tree_view *v) {
       !v->cmx || !v->cmy || !v->cmz || !v->size || !v->left || !v->right || !v->perm)
     return fail(KD_ERR_ARG, "a node array is NULL");
   HIP_TRY(hipSetDevice(c->device));
-  const int64_t slots = c->shape.slots;
+  const int64_t slots = d.slots;
+  const TreeDev tr = tree_at(c->tree, d.node_off);
   HostTree t;
   t.leaf = leaf;
   t.start.resize(slots);
@@ -663,14 +720,15 @@ int kd_export_tree(kd_ctx *c, int32_t layout, kd_tree_view *v) {
   t.rec.resize(slots);
   std::vector<uint32_t> oid(n);
   hipStream_t s = c->stream;
-  HIP_TRY(hipMemcpyAsync(t.start.data(), c->tree.seg_start, 4 * slots, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(t.count.data(), c->tree.seg_count, 4 * slots, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(t.sd.data(), c->tree.split_dim, slots, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(t.sv.data(), c->tree.split_val, 8 * slots, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(t.size.data(), c->tree.size, 8 * slots, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(t.rec.data(), c->tree.rec64, sizeof(NodeRec64) * slots,
+  HIP_TRY(hipMemcpyAsync(t.start.data(), tr.seg_start, 4 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.count.data(), tr.seg_count, 4 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.sd.data(), tr.split_dim, slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.sv.data(), tr.split_val, 8 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.size.data(), tr.size, 8 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.rec.data(), tr.rec64, sizeof(NodeRec64) * slots,
                          hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(oid.data(), c->st[c->cur].oid, 4 * n, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(oid.data(), c->st[c->cur].oid + d.body_off, 4 * n,
+                         hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   for (int64_t i = 0; i < need; ++i) {
     v->is_leaf[i] = 1;
@@ -680,7 +738,7 @@ int kd_export_tree(kd_ctx *c, int32_t layout, kd_tree_view *v) {
     v->split_val[i] = v->m[i] = v->cmx[i] = v->cmy[i] = v->cmz[i] = v->size[i] = 0.0;
     v->left[i] = v->right[i] = 0;
   }
-  for (int64_t i = 0; i < n; ++i) v->perm[i] = oid[i];
+  for (int64_t i = 0; i < n; ++i) v->perm[i] = (int64_t)oid[i] - (int64_t)d.body_off;
   if (layout == KD_LAYOUT_PADDED) {
     if (!emit_padded(t, 0, 0, need, v))
       return fail(KD_ERR_CAPACITY, "padded layout overflows its %lld-node allocation",
@@ -690,6 +748,12 @@ int kd_export_tree(kd_ctx *c, int32_t layout, kd_tree_view *v) {
   }
   v->n_nodes = need;
   return KD_OK;
+}
+
+int kd_export_tree(kd_ctx *c, int32_t layout, kd_tree_view *v) {
+  if (!c) return fail(KD_ERR_ARG, "NULL argument");
+  if (!c->has_tree) return fail(KD_ERR_STATE, "no tree built");
+  return export_impl(c, c->hsys[0], layout, v);
 }
 
 // ---------------------------------------------------------------------------
@@ -868,4 +932,409 @@ extern "C" int kd_accel_tree(int64_t n, const double *x, const double *y, const 
   if (e != hipSuccess) return fail(KD_ERR_HIP, "kd_accel_tree: %s", hipGetErrorString(e));
   if (herr) return fail(KD_ERR_ARG, "tree deeper than the traversal stack allows");
   return KD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches: many independent systems built and stepped together
+// ---------------------------------------------------------------------------
+
+struct kd_batch {
+  kd_ctx *c = nullptr;              // buffers for every system (padded layout)
+  std::vector<int64_t> n;           // bodies per system
+  int64_t n_total = 0;              // sum of n (the caller's concatenated layout)
+  uint2 *tasks = nullptr;           // (system, first body) per wavefront, largest systems first
+  uint32_t ntasks = 0;
+  uint32_t *queue = nullptr;        // task counter of the persistent walk
+  int cus = 0;
+  WalkShape ws;
+  bool use_graph = true;
+  hipGraph_t graph[2] = {nullptr, nullptr};  // one step from state buffer 0 / 1
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+};
+
+static void drop_graphs(kd_batch *b) {
+  for (int i = 0; i < 2; ++i) {
+    if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
+    if (b->graph[i]) (void)hipGraphDestroy(b->graph[i]);
+    b->exec[i] = nullptr;
+    b->graph[i] = nullptr;
+  }
+}
+
+// Padded body position of caller index i of system s.
+static void batch_to_padded(const kd_batch *b, const double *src, std::vector<double> &dst) {
+  dst.assign(b->c->forest.total, 0.0);
+  int64_t at = 0;
+  for (size_t s = 0; s < b->n.size(); ++s) {
+    const uint32_t off = b->c->hsys[s].body_off;
+    std::memcpy(&dst[off], src + at, sizeof(double) * (size_t)b->n[s]);
+    at += b->n[s];
+  }
+}
+
+static void batch_from_padded(const kd_batch *b, const std::vector<double> &src, double *dst) {
+  int64_t at = 0;
+  for (size_t s = 0; s < b->n.size(); ++s) {
+    const uint32_t off = b->c->hsys[s].body_off;
+    std::memcpy(dst + at, &src[off], sizeof(double) * (size_t)b->n[s]);
+    at += b->n[s];
+  }
+}
+
+static int batch_traverse(kd_batch *b) {
+  kd_ctx *c = b->c;
+  return launch_traverse_forest(c->params.precision, c->params.count_interactions != 0, b->ws,
+                                c->tree, c->pos64, c->pos32, c->forest, b->tasks, b->ntasks,
+                                b->queue, b->cus, c->params.theta, c->params.leaf_size,
+                                c->acc[0], c->acc[1], c->acc[2], c->counters,
+                                c->stream) == hipSuccess
+             ? KD_OK
+             : fail(KD_ERR_HIP, "forest walk launch failed: %s",
+                    hipGetErrorString(hipGetLastError()));
+}
+
+// One step (build, walk, kick-drift) enqueued on the context's stream.
+static int batch_step_enqueue(kd_batch *b) {
+  kd_ctx *c = b->c;
+  int rc = build_impl(c);
+  if (rc) return rc;
+  if ((rc = batch_traverse(b))) return rc;
+  HIP_TRY(launch_kick_drift(c->st[c->cur], c->acc[0], c->acc[1], c->acc[2], c->forest.total,
+                            c->params.dt, c->bad, c->stream));
+  return KD_OK;
+}
+
+// Capture the step that starts from state buffer `from` into a graph.  A
+// captured memset node is refused: the step clears memory with kernels only.
+static int batch_capture(kd_batch *b, int from) {
+  kd_ctx *c = b->c;
+  const int saved = c->cur;
+  c->cur = from;
+  HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+  int rc = batch_step_enqueue(b);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->cur = saved;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(KD_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+  size_t nn = 0;
+  e = hipGraphGetNodes(g, nullptr, &nn);
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (e == hipSuccess && nn) e = hipGraphGetNodes(g, nodes.data(), &nn);
+  for (size_t i = 0; e == hipSuccess && i < nn; ++i) {
+    hipGraphNodeType ty;
+    e = hipGraphNodeGetType(nodes[i], &ty);
+    if (e == hipSuccess && ty == hipGraphNodeTypeMemset) {
+      (void)hipGraphDestroy(g);
+      return fail(KD_ERR_HIP, "the captured step holds a memset node");
+    }
+  }
+  hipGraphExec_t x = nullptr;
+  if (e == hipSuccess) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(KD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
+  }
+  b->graph[from] = g;
+  b->exec[from] = x;
+  return KD_OK;
+}
+
+extern "C" int kd_batch_destroy(kd_batch *b) {
+  if (!b) return KD_OK;
+  if (b->c) {
+    (void)hipSetDevice(b->c->device);
+    if (b->c->stream) (void)hipStreamSynchronize(b->c->stream);
+  }
+  drop_graphs(b);
+  if (b->tasks) (void)hipFree(b->tasks);
+  if (b->queue) (void)hipFree(b->queue);
+  if (b->c) kd_destroy(b->c);
+  delete b;
+  return KD_OK;
+}
+
+extern "C" int kd_batch_create(int64_t nsys, const int64_t *n, const kd_params *params,
+                               kd_batch **out) {
+  if (!out || !n) return fail(KD_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (nsys < 1 || nsys > 65535) return fail(KD_ERR_ARG, "nsys must be in [1, 65535]");
+  kd_params p;
+  if (params) p = *params;
+  else kd_default_params(&p);
+  int rc = check_params(&p);
+  if (rc) return rc;
+  // layout: systems at 64-aligned body offsets, heaps back to back
+  std::vector<SysDesc> hs((size_t)nsys);
+  uint64_t body = 0, slots = 0;
+  int64_t total_n = 0;
+  for (int64_t i = 0; i < nsys; ++i) {
+    if (n[i] < 1 || n[i] > 0x7fffffffLL) return fail(KD_ERR_ARG, "n[%lld] out of range", (long long)i);
+    const Shape sh = make_shape(n[i], p.leaf_size);
+    hs[i] = make_sys(sh, (uint32_t)body, (uint32_t)slots);
+    body += ((uint64_t)n[i] + 63) & ~63ull;
+    slots += (uint64_t)sh.slots;
+    total_n += n[i];
+    if (body > 0x7fff0000ull || slots > 0xffff0000ull)
+      return fail(KD_ERR_CAPACITY, "batch too large for 32-bit body/node indices");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(KD_ERR_NODEVICE, "no HIP device visible");
+  kd_batch *b = new kd_batch();
+  b->n.assign(n, n + nsys);
+  b->n_total = total_n;
+  kd_ctx *c = new kd_ctx();
+  b->c = c;
+  c->params = p;
+  c->cap = (int64_t)body;
+  Forest f;
+  f.nsys = (int)nsys;
+  for (const SysDesc &d : hs) {
+    f.max_sub_level = std::max(f.max_sub_level, (int)d.sub_level);
+    f.max_n = std::max(f.max_n, d.n);
+  }
+  Sizing z{sort_temp_bytes((uint32_t)body, (int)nsys), level_cnt_entries(f), tieflag_entries(f),
+           (int)nsys};
+  auto bail = [&](int code) {
+    std::string msg = g_err;
+    kd_batch_destroy(b);
+    g_err = msg;
+    return code;
+  };
+  if ((rc = create_impl(c, (int64_t)body, z))) return bail(rc);
+  if ((rc = alloc_nodes(c, (int64_t)slots))) return bail(rc);
+  c->hsys = hs;
+  if ((rc = install_forest(c))) return bail(rc);
+  c->n = (int64_t)body;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) {
+    fail(KD_ERR_HIP, "hipGetDeviceProperties failed");
+    return bail(KD_ERR_HIP);
+  }
+  b->cus = prop.multiProcessorCount;
+  // wavefront tasks, largest systems first (their walks are the longest)
+  std::vector<int64_t> order((size_t)nsys);
+  for (int64_t i = 0; i < nsys; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t q) { return n[a] > n[q]; });
+  std::vector<uint2> tasks;
+  for (int64_t i : order)
+    for (int64_t w = 0; w < n[i]; w += 64) tasks.push_back(make_uint2((uint32_t)i, (uint32_t)w));
+  b->ntasks = (uint32_t)tasks.size();
+  if (hipMalloc(&b->tasks, sizeof(uint2) * tasks.size()) != hipSuccess ||
+      hipMalloc(&b->queue, sizeof(uint32_t)) != hipSuccess) {
+    fail(KD_ERR_NOMEM, "task table allocation failed");
+    return bail(KD_ERR_NOMEM);
+  }
+  if (hipMemcpy(b->tasks, tasks.data(), sizeof(uint2) * tasks.size(), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    fail(KD_ERR_HIP, "task table upload failed");
+    return bail(KD_ERR_HIP);
+  }
+  *out = b;
+  return KD_OK;
+}
+
+extern "C" int64_t kd_batch_size(const kd_batch *b) { return b ? b->n_total : -1; }
+
+extern "C" int kd_batch_upload(kd_batch *b, const double *x, const double *y, const double *z,
+                               const double *vx, const double *vy, const double *vz,
+                               const double *m) {
+  if (!b) return fail(KD_ERR_ARG, "batch is NULL");
+  const double *src[7] = {x, y, z, vx, vy, vz, m};
+  for (int i = 0; i < 7; ++i)
+    if (!src[i]) return fail(KD_ERR_ARG, "input array %d is NULL", i);
+  kd_ctx *c = b->c;
+  HIP_TRY(hipSetDevice(c->device));
+  const uint32_t total = c->forest.total;
+  // both state buffers: zero bodies in the padding, positions as original
+  // indices (padding bodies never enter a tree and stay where they are)
+  for (int k = 0; k < 2; ++k) {
+    StateDev &s = c->st[k];
+    double *dst[7] = {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m};
+    for (int i = 0; i < 7; ++i)
+      HIP_TRY(hipMemsetAsync(dst[i], 0, sizeof(double) * total, c->stream));
+    HIP_TRY(launch_iota(s.oid, total, c->stream));
+  }
+  for (int d = 0; d < 3; ++d) HIP_TRY(hipMemsetAsync(c->acc[d], 0, sizeof(double) * total, c->stream));
+  StateDev &s = c->st[c->cur];
+  double *dst[7] = {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m};
+  std::vector<double> stage;
+  for (int i = 0; i < 7; ++i) {
+    batch_to_padded(b, src[i], stage);
+    HIP_TRY(hipMemcpyAsync(dst[i], stage.data(), sizeof(double) * total, hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));  // stage is reused
+  }
+  HIP_TRY(hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
+  HIP_TRY(launch_sanitize(s, total, c->bad, c->stream));
+  int bad = 0;
+  HIP_TRY(hipMemcpyAsync(&bad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (bad) {
+    c->has_state = false;
+    return fail(KD_ERR_NONFINITE, "non-finite position, velocity or mass in upload");
+  }
+  c->has_state = true;
+  c->has_tree = false;
+  return KD_OK;
+}
+
+static int batch_fetch(kd_batch *b, const double *const src[3], double *const out[3]) {
+  kd_ctx *c = b->c;
+  const uint32_t total = c->forest.total;
+  HIP_TRY(launch_to_original(c->st[c->cur].oid, src[0], src[1], src[2], c->acc_o[0], c->acc_o[1],
+                             c->acc_o[2], total, c->stream));
+  std::vector<double> stage(total);
+  for (int k = 0; k < 3; ++k) {
+    if (!out[k]) continue;
+    HIP_TRY(hipMemcpyAsync(stage.data(), c->acc_o[k], sizeof(double) * total,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    batch_from_padded(b, stage, out[k]);
+  }
+  return KD_OK;
+}
+
+extern "C" int kd_batch_download(kd_batch *b, double *x, double *y, double *z, double *vx,
+                                 double *vy, double *vz) {
+  if (!b) return fail(KD_ERR_ARG, "batch is NULL");
+  kd_ctx *c = b->c;
+  if (!c->has_state) return fail(KD_ERR_STATE, "no bodies uploaded");
+  HIP_TRY(hipSetDevice(c->device));
+  StateDev &s = c->st[c->cur];
+  const double *p[3] = {s.x, s.y, s.z}, *v[3] = {s.vx, s.vy, s.vz};
+  double *po[3] = {x, y, z}, *vo[3] = {vx, vy, vz};
+  int rc = batch_fetch(b, p, po);
+  if (rc) return rc;
+  return batch_fetch(b, v, vo);
+}
+
+extern "C" int kd_batch_accel(kd_batch *b, double *ax, double *ay, double *az) {
+  if (!b) return fail(KD_ERR_ARG, "batch is NULL");
+  kd_ctx *c = b->c;
+  if (!c->has_state) return fail(KD_ERR_STATE, "no bodies uploaded");
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = build_impl(c);
+  if (rc) return rc;
+  if (c->params.count_interactions)
+    HIP_TRY(hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream));
+  if ((rc = batch_traverse(b))) return rc;
+  const double *a[3] = {c->acc[0], c->acc[1], c->acc[2]};
+  double *o[3] = {ax, ay, az};
+  return batch_fetch(b, a, o);
+}
+
+extern "C" int kd_batch_step(kd_batch *b, int64_t steps) {
+  if (!b) return fail(KD_ERR_ARG, "batch is NULL");
+  kd_ctx *c = b->c;
+  if (!c->has_state) return fail(KD_ERR_STATE, "no bodies uploaded");
+  if (steps < 0) return fail(KD_ERR_ARG, "steps must be >= 0");
+  HIP_TRY(hipSetDevice(c->device));
+  HIP_TRY(hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
+  if (c->params.count_interactions)
+    HIP_TRY(hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream));
+  int rc;
+  for (int64_t s = 0; s < steps; ++s) {
+    if (b->use_graph) {
+      const int from = c->cur;
+      if (!b->exec[from] && (rc = batch_capture(b, from))) return rc;
+      HIP_TRY(hipGraphLaunch(b->exec[from], c->stream));
+      c->cur ^= 1;
+      c->has_tree = true;
+    } else if ((rc = batch_step_enqueue(b))) {
+      return rc;
+    }
+  }
+  int bad = 0;
+  HIP_TRY(hipMemcpyAsync(&bad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (bad) return fail(KD_ERR_NONFINITE, "a body reached a non-finite position");
+  return KD_OK;
+}
+
+extern "C" int kd_batch_export_tree(kd_batch *b, int64_t sys, int32_t layout, kd_tree_view *v) {
+  if (!b) return fail(KD_ERR_ARG, "batch is NULL");
+  if (sys < 0 || sys >= (int64_t)b->n.size()) return fail(KD_ERR_ARG, "system index out of range");
+  if (!b->c->has_tree) return fail(KD_ERR_STATE, "no tree built");
+  return export_impl(b->c, b->c->hsys[(size_t)sys], layout, v);
+}
+
+extern "C" int kd_batch_set_tuning(kd_batch *b, const char *key, int64_t value) {
+  if (!b || !key) return fail(KD_ERR_ARG, "NULL argument");
+  const std::string k = key;
+  if (k == "graph") {
+    b->use_graph = value != 0;
+  } else if (k == "walk_threads") {
+    if (value != 64 && value != 128 && value != 256 && value != 512)
+      return fail(KD_ERR_ARG, "walk_threads must be 64, 128, 256 or 512");
+    b->ws.threads = (int)value;
+  } else if (k == "walk_waves_per_cu") {
+    if (value < 0 || value > 64) return fail(KD_ERR_ARG, "walk_waves_per_cu must be in [0, 64]");
+    b->ws.waves_per_cu = (int)value;
+  } else {
+    return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
+  }
+  drop_graphs(b);  // the captured steps embed the launch shape
+  return KD_OK;
+}
+
+extern "C" int kd_batch_autotune(kd_batch *b, int64_t reps, double out[3]) {
+  if (!b) return fail(KD_ERR_ARG, "batch is NULL");
+  kd_ctx *c = b->c;
+  if (!c->has_state) return fail(KD_ERR_STATE, "no bodies uploaded");
+  if (reps < 1) reps = 1;
+  HIP_TRY(hipSetDevice(c->device));
+  int rc = build_impl(c);  // the walk needs a tree; positions are unchanged
+  if (rc) return rc;
+  const int saved_count = c->params.count_interactions;
+  c->params.count_interactions = 0;
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  const int threads[4] = {64, 128, 256, 512};
+  const int wpc[5] = {0, 8, 16, 24, 32};
+  WalkShape best = b->ws;
+  float best_ms = 1e30f;
+  for (int ti = 0; ti < 4 && rc == KD_OK; ++ti) {
+    for (int wi = 0; wi < 5 && rc == KD_OK; ++wi) {
+      b->ws.threads = threads[ti];
+      b->ws.waves_per_cu = wpc[wi];
+      if ((rc = batch_traverse(b))) break;  // warm-up
+      float ms_min = 1e30f;
+      for (int64_t r = 0; r < reps && rc == KD_OK; ++r) {
+        (void)hipEventRecord(e0, c->stream);
+        rc = batch_traverse(b);
+        (void)hipEventRecord(e1, c->stream);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = fail(KD_ERR_HIP, "event sync failed");
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms_min = std::min(ms_min, ms);
+      }
+      if (rc == KD_OK && ms_min < best_ms) {
+        best_ms = ms_min;
+        best = b->ws;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->params.count_interactions = saved_count;
+  b->ws = best;
+  drop_graphs(b);
+  if (rc) return rc;
+  if (out) {
+    out[0] = best.threads;
+    out[1] = best.waves_per_cu;
+    out[2] = best_ms;
+  }
+  return KD_OK;
+}
+
+extern "C" int kd_batch_counters(kd_batch *b, int64_t out[5]) {
+  if (!b) return fail(KD_ERR_ARG, "batch is NULL");
+  return kd_counters(b->c, out);
 }

@@ -13,9 +13,14 @@
 //      emits leaves in canonical order (ascending original index), node
 //      moments bottom-up, and writes the body state in tree order;
 //   4. one workgroup reduces the moments of the device-wide levels.
+// A forest (many independent systems) runs the same kernels with the system
+// index in blockIdx.y; see SysDesc.
 // Ties on a split coordinate are resolved by original index, lazily: only a
 // run of equal keys that straddles the median is reordered.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
+#include <algorithm>
 
 #include "kd_internal.hpp"
 
@@ -130,10 +135,14 @@ __device__ __forceinline__ SegV block_seg_carry(SegV agg, uint64_t *shv,
 
 // Segment of every heap slot: descend from the root splitting at floor(c/2);
 // slots below a leaf get count 0.
-__global__ void k_shape(uint32_t *seg_start, uint32_t *seg_count, int64_t slots,
-                        uint32_t n, int leaf) {
+__global__ void k_shape(uint32_t *seg_start, uint32_t *seg_count, const SysDesc *sys,
+                        int leaf) {
+  const SysDesc S = sys[blockIdx.y];
   int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= slots) return;
+  if (h >= S.slots) return;
+  seg_start += S.node_off;
+  seg_count += S.node_off;
+  const uint32_t n = S.n;
   uint32_t hp = (uint32_t)h + 1;
   int lev = 31 - __clz(hp);
   uint32_t k = hp - (1u << lev);
@@ -179,13 +188,16 @@ struct Coords {
   const double *c[3];
 };
 
-__global__ void k_level_split(Lists L, Coords P, TreeDev t, uint32_t level,
-                              uint32_t *tieflag) {
+__global__ void k_level_split(Lists L, Coords P, TreeDev t0, const SysDesc *sys,
+                              uint32_t level, uint32_t *tieflag, uint32_t tf_stride) {
+  const SysDesc S = sys[blockIdx.y];
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t nseg = 1u << level;
-  if (k >= nseg) return;
+  if (k >= nseg || level >= S.sub_level) return;
+  const TreeDev t = tree_at(t0, S.node_off);
+  tieflag += blockIdx.y * tf_stride;
   uint32_t h = nseg - 1 + k;
-  uint32_t s = t.seg_start[h], c = t.seg_count[h];
+  uint32_t s = t.seg_start[h] + S.body_off, c = t.seg_count[h];
   double ext[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -210,15 +222,19 @@ __global__ void k_level_split(Lists L, Coords P, TreeDev t, uint32_t level,
 // flagged segment: radix-select the threshold index, then stably partition
 // the run (scratch = the same list's output buffer, overwritten later).
 __global__ __launch_bounds__(1024) void k_level_tiefix(Lists L, Lists scratch,
-                                                       Coords P, TreeDev t,
+                                                       Coords P, TreeDev t0,
+                                                       const SysDesc *sys,
                                                        const uint32_t *oid,
                                                        uint32_t level,
-                                                       const uint32_t *tieflag) {
+                                                       const uint32_t *tieflag,
+                                                       uint32_t tf_stride) {
   constexpr int NT = 1024;
+  const SysDesc S = sys[blockIdx.y];
   uint32_t k = blockIdx.x;
-  if (!tieflag[k]) return;
+  if (level >= S.sub_level || !tieflag[blockIdx.y * tf_stride + k]) return;
+  const TreeDev t = tree_at(t0, S.node_off);
   uint32_t h = (1u << level) - 1 + k;
-  uint32_t s = t.seg_start[h], c = t.seg_count[h];
+  uint32_t s = t.seg_start[h] + S.body_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
   uint32_t *ls = L.l[sd];
   uint32_t *tmp = scratch.l[sd];
@@ -293,13 +309,17 @@ __global__ __launch_bounds__(1024) void k_level_tiefix(Lists L, Lists scratch,
   for (uint32_t p = r0 + threadIdx.x; p < r1; p += NT) ls[p] = tmp[p];
 }
 
-__global__ __launch_bounds__(LVL_THREADS) void k_level_flags(Lists L, TreeDev t,
+__global__ __launch_bounds__(LVL_THREADS) void k_level_flags(Lists L, TreeDev t0,
+                                                             const SysDesc *sys,
                                                              uint32_t level,
                                                              uint32_t T,
                                                              uint8_t *flags) {
+  const SysDesc S = sys[blockIdx.y];
+  if (level >= S.sub_level) return;
+  const TreeDev t = tree_at(t0, S.node_off);
   uint32_t k = blockIdx.x / T, q = blockIdx.x % T;
   uint32_t h = (1u << level) - 1 + k;
-  uint32_t s = t.seg_start[h], c = t.seg_count[h];
+  uint32_t s = t.seg_start[h] + S.body_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
   uint32_t mid = s + c / 2, e = s + c;
   const uint32_t *ls = L.l[sd];
@@ -310,16 +330,21 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_flags(Lists L, TreeDev t,
   }
 }
 
-__global__ __launch_bounds__(LVL_THREADS) void k_level_count(Lists L, TreeDev t,
+__global__ __launch_bounds__(LVL_THREADS) void k_level_count(Lists L, TreeDev t0,
+                                                             const SysDesc *sys,
                                                              uint32_t level,
                                                              uint32_t T,
                                                              const uint8_t *flags,
                                                              uint32_t *cnt) {
   __shared__ uint32_t sh[LVL_THREADS / 64];
+  const SysDesc S = sys[blockIdx.y];
+  if (level >= S.sub_level) return;
+  const TreeDev t = tree_at(t0, S.node_off);
   uint32_t nseg = 1u << level;
+  cnt += (size_t)blockIdx.y * 3 * nseg * T;
   uint32_t k = blockIdx.x / T, q = blockIdx.x % T;
   uint32_t h = nseg - 1 + k;
-  uint32_t s = t.seg_start[h], c = t.seg_count[h];
+  uint32_t s = t.seg_start[h] + S.body_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
   uint32_t e = s + c;
   uint32_t base = s + q * LVL_TILE;
@@ -339,9 +364,12 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_count(Lists L, TreeDev t,
 }
 
 // Exclusive scan over the T tile counts of one (list, segment).
-__global__ __launch_bounds__(256) void k_level_scan(uint32_t *cnt, uint32_t T) {
+__global__ __launch_bounds__(256) void k_level_scan(uint32_t *cnt, uint32_t T,
+                                                    const SysDesc *sys, uint32_t level) {
   __shared__ uint32_t sh[4];
-  uint32_t *row = cnt + (size_t)blockIdx.x * T;
+  if (level >= sys[blockIdx.y].sub_level) return;
+  const uint32_t rows = 3u << level;
+  uint32_t *row = cnt + ((size_t)blockIdx.y * rows + blockIdx.x) * T;
   uint32_t carry = 0;
   for (uint32_t b = 0; b < T; b += 256) {
     uint32_t i = b + threadIdx.x;
@@ -354,16 +382,21 @@ __global__ __launch_bounds__(256) void k_level_scan(uint32_t *cnt, uint32_t T) {
 }
 
 __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(Lists L, Lists Lout,
-                                                               TreeDev t,
+                                                               TreeDev t0,
+                                                               const SysDesc *sys,
                                                                uint32_t level,
                                                                uint32_t T,
                                                                const uint8_t *flags,
                                                                const uint32_t *cnt) {
   __shared__ uint32_t sh[LVL_THREADS / 64];
+  const SysDesc S = sys[blockIdx.y];
+  if (level >= S.sub_level) return;
+  const TreeDev t = tree_at(t0, S.node_off);
   uint32_t nseg = 1u << level;
+  cnt += (size_t)blockIdx.y * 3 * nseg * T;
   uint32_t k = blockIdx.x / T, q = blockIdx.x % T;
   uint32_t h = nseg - 1 + k;
-  uint32_t s = t.seg_start[h], c = t.seg_count[h];
+  uint32_t s = t.seg_start[h] + S.body_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
   uint32_t mid = s + c / 2, e = s + c;
   uint32_t base = s + q * LVL_TILE + threadIdx.x * LVL_ITEMS;
@@ -407,7 +440,8 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(Lists L, Lists Lo
 // ---------------------------------------------------------------------------
 
 struct SubArgs {
-  Lists L;          // sorted lists (device-wide levels done)
+  Lists L[2];       // sorted lists after an even / odd number of device-wide levels
+  const SysDesc *sys;
   Coords P;         // positions, working order
   const double *m;  // masses, working order
   StateDev in;      // state, working order
@@ -417,7 +451,6 @@ struct SubArgs {
   uint32_t *perm;   // tree position -> working id
   double4 *pos64;   // tree order (x, y, z, m)
   float4 *pos32;
-  uint32_t level;
   int leaf;
   double theta2;
 };
@@ -512,15 +545,22 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   __shared__ uint64_t shv[NT / 64];
   __shared__ uint32_t shh[NT / 64];
 
+  const SysDesc S = a.sys[blockIdx.y];
   const uint32_t k = blockIdx.x;
-  const uint32_t h0 = (1u << a.level) - 1 + k;
-  const uint32_t s0 = a.t.seg_start[h0], n0 = a.t.seg_count[h0];
+  const uint32_t level = S.sub_level;
+  if (k >= (1u << level)) return;
+  const TreeDev t = tree_at(a.t, S.node_off);
+  const Lists L = a.L[level & 1];
+  const uint32_t h0 = (1u << level) - 1 + k;
+  // s0l: first body of the segment within its system; s0: in the body arrays
+  const uint32_t s0l = t.seg_start[h0], n0 = t.seg_count[h0];
+  const uint32_t s0 = s0l + S.body_off;
   if (n0 == 0) return;
   const int leaf = a.leaf;
   const int tid = threadIdx.x;
 
   for (uint32_t i = tid; i < n0; i += NT) {
-    uint32_t id = a.L.l[0][s0 + i];
+    uint32_t id = L.l[0][s0 + i];
     ids[i] = id;
     lst[0][0][i] = (uint16_t)i;
     a.loc[id] = i;
@@ -528,8 +568,8 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   __threadfence_block();
   __syncthreads();
   for (uint32_t i = tid; i < n0; i += NT) {
-    lst[0][1][i] = (uint16_t)a.loc[a.L.l[1][s0 + i]];
-    lst[0][2][i] = (uint16_t)a.loc[a.L.l[2][s0 + i]];
+    lst[0][1][i] = (uint16_t)a.loc[L.l[1][s0 + i]];
+    lst[0][2][i] = (uint16_t)a.loc[L.l[2][s0 + i]];
   }
   __syncthreads();
 
@@ -547,7 +587,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   int cur = 0;
   int lam_end = 0;
   for (int lam = 0;; ++lam) {
-    const uint32_t hbase = (1u << (a.level + lam)) - 1 + (k << lam);
+    const uint32_t hbase = (1u << (level + lam)) - 1 + (k << lam);
     // one thread per segment of this level: split decision or leaf mark
     for (uint32_t c = tid; c < (1u << lam); c += NT) {
       uint32_t sgs = 0, sgc = n0;
@@ -600,9 +640,9 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
             ls[y + 1] = v;
           }
         }
-        a.t.split_dim[h] = (uint8_t)sd;
-        a.t.split_val[h] = kv;
-        a.t.size[h] = ext[sd];
+        t.split_dim[h] = (uint8_t)sd;
+        t.split_val[h] = kv;
+        t.size[h] = ext[sd];
         sdarr[sgs] = (uint8_t)sd;
       }
     }
@@ -751,18 +791,18 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
       mz += b.w * b.z;
     }
     const uint32_t h = hl[it];
-    a.t.mm[h] = mm;
-    a.t.mx[h] = mx;
-    a.t.my[h] = my;
-    a.t.mz[h] = mz;
-    write_leaf_rec(a.t, h, s0 + st[it], ct[it]);
+    t.mm[h] = mm;
+    t.mx[h] = mx;
+    t.my[h] = my;
+    t.mz[h] = mz;
+    write_leaf_rec(t, h, s0l + st[it], ct[it]);
   }
 
   // moments bottom-up over this subtree's internal nodes
   __threadfence_block();
   __syncthreads();
   for (int lam = lam_end - 1; lam >= 0; --lam) {
-    const uint32_t hbase = (1u << (a.level + lam)) - 1 + (k << lam);
+    const uint32_t hbase = (1u << (level + lam)) - 1 + (k << lam);
     for (uint32_t c = tid; c < (1u << lam); c += NT) {
       uint32_t s = 0, cnt = n0;
       bool ok = true;
@@ -782,11 +822,11 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
       if (!ok || cnt <= (uint32_t)leaf) continue;
       uint32_t h = hbase + c;
       uint32_t l = 2 * h + 1, r = 2 * h + 2;
-      a.t.mm[h] = a.t.mm[l] + a.t.mm[r];
-      a.t.mx[h] = a.t.mx[l] + a.t.mx[r];
-      a.t.my[h] = a.t.my[l] + a.t.my[r];
-      a.t.mz[h] = a.t.mz[l] + a.t.mz[r];
-      write_internal_rec(a.t, h, s0 + s, cnt, a.theta2);
+      t.mm[h] = t.mm[l] + t.mm[r];
+      t.mx[h] = t.mx[l] + t.mx[r];
+      t.my[h] = t.my[l] + t.my[r];
+      t.mz[h] = t.mz[l] + t.mz[r];
+      write_internal_rec(t, h, s0l + s, cnt, a.theta2);
     }
     __threadfence_block();
     __syncthreads();
@@ -794,9 +834,12 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
 }
 
 // Moments and records of the device-wide levels [0, level), bottom-up.
-__global__ __launch_bounds__(1024) void k_moments_top(TreeDev t, uint32_t level,
+__global__ __launch_bounds__(1024) void k_moments_top(TreeDev t0, const SysDesc *sys,
                                                       double theta2) {
 #pragma clang fp contract(off)
+  const SysDesc S = sys[blockIdx.x];
+  const TreeDev t = tree_at(t0, S.node_off);
+  const uint32_t level = S.sub_level;
   for (int lev = (int)level - 1; lev >= 0; --lev) {
     uint32_t nseg = 1u << lev;
     for (uint32_t k = threadIdx.x; k < nseg; k += blockDim.x) {
@@ -840,53 +883,102 @@ Shape make_shape(int64_t n, int leaf) {
 }
 
 
-size_t sort_temp_bytes(uint32_t n) {
+SysDesc make_sys(const Shape &sh, uint32_t body_off, uint32_t node_off) {
+  SysDesc d{};
+  d.body_off = body_off;
+  d.n = (uint32_t)sh.n;
+  d.node_off = node_off;
+  d.slots = (uint32_t)sh.slots;
+  d.sub_level = (uint32_t)sh.sub_level;
+  d.levels = (uint32_t)sh.levels;
+  return d;
+}
+
+size_t sort_temp_bytes(uint32_t n, int nsys) {
   size_t bytes = 0;
-  (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
-                            (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 64);
+  if (nsys <= 1) {
+    (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                    (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 64);
+  } else {
+    (void)rocprim::segmented_radix_sort_pairs(
+        nullptr, bytes, (uint64_t *)nullptr, (uint64_t *)nullptr, (uint32_t *)nullptr,
+        (uint32_t *)nullptr, n, (unsigned)nsys, (const uint32_t *)nullptr,
+        (const uint32_t *)nullptr, 0, 64);
+  }
   return bytes;
 }
 
-hipError_t launch_shape(const TreeDev &t, const Shape &sh, hipStream_t s) {
-  int64_t blocks = (sh.slots + 255) / 256;
-  k_shape<<<dim3((unsigned)blocks), dim3(256), 0, s>>>(t.seg_start, t.seg_count, sh.slots,
-                                                      (uint32_t)sh.n, sh.leaf);
+// Tiles per segment at a device-wide level.
+static uint32_t level_tiles(const Forest &f, int lev) {
+  const uint32_t nseg = 1u << lev;
+  const uint32_t maxseg = (f.max_n + nseg - 1) / nseg;
+  return (maxseg + LVL_TILE - 1) / LVL_TILE;
+}
+
+size_t level_cnt_entries(const Forest &f) {
+  size_t m = 1;
+  for (int lev = 0; lev < f.max_sub_level; ++lev)
+    m = std::max(m, (size_t)f.nsys * 3 * ((size_t)1 << lev) * level_tiles(f, lev));
+  return m;
+}
+
+size_t tieflag_entries(const Forest &f) { return (size_t)f.nsys << f.max_sub_level; }
+
+hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t s) {
+  const unsigned blocks = (f.max_slots + 255) / 256;
+  k_shape<<<dim3(blocks, (unsigned)f.nsys), dim3(256), 0, s>>>(t.seg_start, t.seg_count, f.sys,
+                                                               leaf);
   return hipGetLastError();
 }
 
-hipError_t launch_build(const Shape &sh, const StateDev &in, const StateDev &out,
-                        const TreeDev &t, BuildBufs &b, double4 *pos64,
-                        float4 *pos32, double theta, hipStream_t s) {
-  const uint32_t n = (uint32_t)sh.n;
+hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
+                        const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
+                        double theta, hipStream_t s) {
   const double *coord[3] = {in.x, in.y, in.z};
+  const unsigned ny = (unsigned)f.nsys;
   hipError_t e;
-  if (sort_temp_bytes(n) > b.sort_tmp_bytes) return hipErrorInvalidValue;
+  // a single system sorts its n bodies (at offset 0); a forest sorts every
+  // system's range as its own segment
+  const uint32_t nsort = f.nsys == 1 ? f.max_n : f.total;
+  if (sort_temp_bytes(nsort, f.nsys) > b.sort_tmp_bytes) return hipErrorInvalidValue;
   for (int d = 0; d < 3; ++d) {
-    k_make_keys<<<(n + 255) / 256, 256, 0, s>>>(coord[d], b.keys_in, b.ids_in, n);
+    k_make_keys<<<(nsort + 255) / 256, 256, 0, s>>>(coord[d], b.keys_in, b.ids_in, nsort);
     size_t bytes = b.sort_tmp_bytes;
-    e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, b.keys_in, b.keys_out, b.ids_in, b.L[d],
-                                  n, 0, 64, s);
+    if (f.nsys == 1)
+      e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, b.keys_in, b.keys_out, b.ids_in, b.L[d],
+                                    nsort, 0, 64, s);
+    else
+      e = rocprim::segmented_radix_sort_pairs(b.sort_tmp, bytes, b.keys_in, b.keys_out, b.ids_in,
+                                              b.L[d], nsort, (unsigned)f.nsys, f.seg_lo,
+                                              f.seg_hi, 0, 64, s);
     if (e != hipSuccess) return e;
   }
   Lists L{{b.L[0], b.L[1], b.L[2]}};
   Lists Lb{{b.Lb[0], b.Lb[1], b.Lb[2]}};
+  const Lists L0 = L, L1 = Lb;
   Coords P{{in.x, in.y, in.z}};
-  for (int lev = 0; lev < sh.sub_level; ++lev) {
-    uint32_t nseg = 1u << lev;
-    uint32_t maxseg = (uint32_t)((sh.n + nseg - 1) / nseg);
-    uint32_t T = (maxseg + LVL_TILE - 1) / LVL_TILE;
-    k_level_split<<<(nseg + 255) / 256, 256, 0, s>>>(L, P, t, lev, b.tieflag);
-    k_level_tiefix<<<nseg, 1024, 0, s>>>(L, Lb, P, t, in.oid, lev, b.tieflag);
-    k_level_flags<<<nseg * T, LVL_THREADS, 0, s>>>(L, t, lev, T, b.flags);
-    k_level_count<<<nseg * T, LVL_THREADS, 0, s>>>(L, t, lev, T, b.flags, b.cnt);
-    k_level_scan<<<3 * nseg, 256, 0, s>>>(b.cnt, T);
-    k_level_scatter<<<nseg * T, LVL_THREADS, 0, s>>>(L, Lb, t, lev, T, b.flags, b.cnt);
+  const uint32_t tf_stride = 1u << f.max_sub_level;
+  for (int lev = 0; lev < f.max_sub_level; ++lev) {
+    const uint32_t nseg = 1u << lev;
+    const uint32_t T = level_tiles(f, lev);
+    k_level_split<<<dim3((nseg + 255) / 256, ny), 256, 0, s>>>(L, P, t, f.sys, lev, b.tieflag,
+                                                                tf_stride);
+    k_level_tiefix<<<dim3(nseg, ny), 1024, 0, s>>>(L, Lb, P, t, f.sys, in.oid, lev, b.tieflag,
+                                                   tf_stride);
+    k_level_flags<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, t, f.sys, lev, T, b.flags);
+    k_level_count<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, t, f.sys, lev, T, b.flags,
+                                                             b.cnt);
+    k_level_scan<<<dim3(3 * nseg, ny), 256, 0, s>>>(b.cnt, T, f.sys, lev);
+    k_level_scatter<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, Lb, t, f.sys, lev, T, b.flags,
+                                                               b.cnt);
     Lists tmp = L;
     L = Lb;
     Lb = tmp;
   }
   SubArgs a;
-  a.L = L;
+  a.L[0] = L0;
+  a.L[1] = L1;
+  a.sys = f.sys;
   a.P = P;
   a.m = in.m;
   a.in = in;
@@ -896,11 +988,10 @@ hipError_t launch_build(const Shape &sh, const StateDev &in, const StateDev &out
   a.perm = b.perm;
   a.pos64 = pos64;
   a.pos32 = pos32;
-  a.level = sh.sub_level;
-  a.leaf = sh.leaf;
+  a.leaf = leaf;
   a.theta2 = theta * theta;
-  k_subtree<<<1u << sh.sub_level, SUB_THREADS, 0, s>>>(a);
-  if (sh.sub_level > 0) k_moments_top<<<1, 1024, 0, s>>>(t, sh.sub_level, theta * theta);
+  k_subtree<<<dim3(1u << f.max_sub_level, ny), SUB_THREADS, 0, s>>>(a);
+  if (f.max_sub_level > 0) k_moments_top<<<ny, 1024, 0, s>>>(t, f.sys, theta * theta);
   return hipGetLastError();
 }
 

@@ -69,6 +69,50 @@ struct TreeDev {
   NodeRecD *recd;
 };
 
+// The same tree with every per-node array advanced by o slots.
+__host__ __device__ inline TreeDev tree_at(const TreeDev &t, uint32_t o) {
+  TreeDev r = t;
+  r.seg_start += o;
+  r.seg_count += o;
+  r.split_dim += o;
+  r.split_val += o;
+  r.size += o;
+  r.mm += o;
+  r.mx += o;
+  r.my += o;
+  r.mz += o;
+  r.rec64 += o;
+  r.rec32 += o;
+  r.recd += o;
+  return r;
+}
+
+// One system of a forest: independent trees built and stepped together.
+// Body arrays hold every system's bodies at [body_off, body_off + n) (body_off
+// a multiple of 64, so no wavefront spans two systems); node arrays hold its
+// heap at [node_off, node_off + slots).  Node records store body positions
+// relative to body_off.  A single system is a forest of one with both
+// offsets 0.
+struct SysDesc {
+  uint32_t body_off;
+  uint32_t n;
+  uint32_t node_off;
+  uint32_t slots;
+  uint32_t sub_level;  // levels split device-wide; the subtree builder takes the next
+  uint32_t levels;
+  uint32_t pad0, pad1;
+};
+
+struct Forest {
+  int nsys = 0;
+  const SysDesc *sys = nullptr;  // device copy, nsys entries
+  int max_sub_level = 0;
+  uint32_t max_n = 0;
+  uint32_t max_slots = 0;
+  uint32_t total = 0;            // body positions, padding included
+  const uint32_t *seg_lo = nullptr, *seg_hi = nullptr;  // device, sort segment bounds
+};
+
 // Body state in "working order" (the previous step's tree order).
 struct StateDev {
   double *x, *y, *z, *vx, *vy, *vz, *m;
@@ -97,15 +141,31 @@ struct BuildBufs {
 };
 
 Shape make_shape(int64_t n, int leaf);
-size_t sort_temp_bytes(uint32_t n);
-hipError_t launch_shape(const TreeDev &t, const Shape &sh, hipStream_t s);
-hipError_t launch_build(const Shape &sh, const StateDev &in, const StateDev &out,
+SysDesc make_sys(const Shape &sh, uint32_t body_off, uint32_t node_off);
+size_t sort_temp_bytes(uint32_t n, int nsys);
+// per-tile count entries and tie flags the device-wide levels of a forest need
+size_t level_cnt_entries(const Forest &f);
+size_t tieflag_entries(const Forest &f);
+hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t s);
+hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
                         double theta, hipStream_t s);
 hipError_t launch_traverse(int precision, bool count, int variant, const TreeDev &t,
                            const double4 *pos64, const float4 *pos32, uint32_t n,
                            double theta, int leaf, double *ax, double *ay, double *az,
                            unsigned long long *counters, hipStream_t s);
+// Launch shape of the forest walk: threads per block (64..512) and resident
+// wavefronts per CU for the persistent form (0: one wavefront per task).
+struct WalkShape {
+  int threads = 256;
+  int waves_per_cu = 0;
+};
+hipError_t launch_traverse_forest(int precision, bool count, const WalkShape &ws,
+                                  const TreeDev &t, const double4 *pos64, const float4 *pos32,
+                                  const Forest &f, const uint2 *tasks, uint32_t ntasks,
+                                  uint32_t *queue, int cus, double theta, int leaf, double *ax,
+                                  double *ay, double *az, unsigned long long *counters,
+                                  hipStream_t s);
 hipError_t launch_kick_drift(const StateDev &s, const double *ax, const double *ay,
                              const double *az, uint32_t n, double dt, int *bad,
                              hipStream_t st);

@@ -19,6 +19,8 @@
 //         terms accumulate in a plain FP32 partial that is folded into the
 //         (sum, compensation) pair after every leaf;
 //   D64   as F32 but the coordinate difference is taken in FP64.
+#include <algorithm>
+
 #include "kd_internal.hpp"
 
 namespace kd {
@@ -276,13 +278,13 @@ __device__ __forceinline__ void leaf_batch(typename P::Acc &acc, const typename 
 #undef KD_P2P
 }
 
+// The walk of one wavefront: bodies [wbase, wbase + 64) of a tree whose
+// records start one before nodes_m1[1] and whose bodies start at bodies[0].
 template <class P, bool COUNT, bool PRED>
-__global__ __launch_bounds__(256) void k_walk(const typename P::Rec *__restrict__ nodes_m1,
-                                              const typename P::Body *__restrict__ bodies,
-                                              WalkArgs<P> a) {
+__device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ nodes_m1,
+                                          const typename P::Body *__restrict__ bodies,
+                                          const WalkArgs<P> &a, uint32_t wbase) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
-  if (wbase >= a.n) return;
   const uint32_t me = wbase + lane;
   const bool valid = me < a.n;
   const typename P::Self p = P::self(bodies[valid ? me : wbase]);
@@ -338,6 +340,47 @@ __global__ __launch_bounds__(256) void k_walk(const typename P::Rec *__restrict_
     }
   }
 }
+
+template <class P, bool COUNT, bool PRED>
+__global__ __launch_bounds__(256) void k_walk(const typename P::Rec *__restrict__ nodes_m1,
+                                              const typename P::Body *__restrict__ bodies,
+                                              WalkArgs<P> a) {
+  const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+  if (wbase >= a.n) return;
+  walk_wave<P, COUNT, PRED>(nodes_m1, bodies, a, wbase);
+}
+
+// Forest walk: a task is one wavefront of one system (x: system, y: first
+// body).  PERSIST: a grid sized to the machine takes tasks from an atomic
+// counter (reset by k_queue_reset before the launch) until none is left;
+// otherwise wavefront w of the grid runs task w.
+template <class P, bool COUNT, bool PERSIST, int NT>
+__global__ __launch_bounds__(NT) void k_walk_forest(const typename P::Rec *__restrict__ nodes,
+                                                    const typename P::Body *__restrict__ bodies,
+                                                    WalkArgs<P> a, const SysDesc *__restrict__ sys,
+                                                    const uint2 *__restrict__ tasks,
+                                                    uint32_t ntasks, uint32_t *queue) {
+  uint32_t t = (blockIdx.x * NT + threadIdx.x) >> 6;
+  for (;;) {
+    if (PERSIST) {
+      uint32_t got = 0;
+      if ((threadIdx.x & 63) == 0) got = atomicAdd(queue, 1u);
+      t = uni(__shfl(got, 0, 64));
+    }
+    if (t >= ntasks) return;
+    const uint2 task = tasks[t];
+    const SysDesc S = sys[task.x];
+    WalkArgs<P> b = a;
+    b.n = S.n;
+    b.ax += S.body_off;
+    b.ay += S.body_off;
+    b.az += S.body_off;
+    walk_wave<P, COUNT, false>(nodes + S.node_off - 1, bodies + S.body_off, b, task.y);
+    if (!PERSIST) return;
+  }
+}
+
+__global__ void k_queue_reset(uint32_t *queue) { *queue = 0u; }
 
 // The plain FP64 path keeps a simpler loop: it is VALU-bound (FP64 sqrt and
 // divide dominate), and this form issues fewer vector instructions per visit
@@ -471,6 +514,74 @@ hipError_t launch_traverse(int precision, bool count, int variant, const TreeDev
   } else {
     WalkArgs<D64> a{t.recd, pos64, n, (uint32_t)leaf, theta2, ax, ay, az, c};
     launch_policy(a, count, variant, s);
+  }
+  return hipGetLastError();
+}
+
+template <class P, bool COUNT, bool PERSIST>
+static void launch_forest_nt(const WalkArgs<P> &a, const WalkShape &ws, const SysDesc *sys,
+                             const uint2 *tasks, uint32_t ntasks, uint32_t *queue, int cus,
+                             hipStream_t s) {
+  const int nt = ws.threads;
+  const unsigned wpb = (unsigned)(nt / 64);
+  unsigned blocks;
+  if (PERSIST)
+    blocks = (unsigned)std::max(1, cus * ws.waves_per_cu / (int)wpb);
+  else
+    blocks = (ntasks + wpb - 1) / wpb;
+#define KD_FOREST(NT)                                                                    \
+  case NT:                                                                               \
+    k_walk_forest<P, COUNT, PERSIST, NT><<<blocks, NT, 0, s>>>(a.nodes, a.bodies, a, sys, \
+                                                              tasks, ntasks, queue);      \
+    break;
+  switch (nt) {
+    KD_FOREST(64)
+    KD_FOREST(128)
+    KD_FOREST(512)
+    default:
+    KD_FOREST(256)
+  }
+#undef KD_FOREST
+}
+
+template <class P>
+static void launch_forest(const WalkArgs<P> &a, bool count, const WalkShape &ws,
+                          const SysDesc *sys, const uint2 *tasks, uint32_t ntasks,
+                          uint32_t *queue, int cus, hipStream_t s) {
+  const bool persist = ws.waves_per_cu > 0;
+  if (persist) k_queue_reset<<<1, 1, 0, s>>>(queue);
+  if (count) {
+    if (persist)
+      launch_forest_nt<P, true, true>(a, ws, sys, tasks, ntasks, queue, cus, s);
+    else
+      launch_forest_nt<P, true, false>(a, ws, sys, tasks, ntasks, queue, cus, s);
+  } else {
+    if (persist)
+      launch_forest_nt<P, false, true>(a, ws, sys, tasks, ntasks, queue, cus, s);
+    else
+      launch_forest_nt<P, false, false>(a, ws, sys, tasks, ntasks, queue, cus, s);
+  }
+}
+
+hipError_t launch_traverse_forest(int precision, bool count, const WalkShape &ws,
+                                  const TreeDev &t, const double4 *pos64, const float4 *pos32,
+                                  const Forest &f, const uint2 *tasks, uint32_t ntasks,
+                                  uint32_t *queue, int cus, double theta, int leaf, double *ax,
+                                  double *ay, double *az, unsigned long long *counters,
+                                  hipStream_t s) {
+  if (ws.threads != 64 && ws.threads != 128 && ws.threads != 256 && ws.threads != 512)
+    return hipErrorInvalidValue;
+  Counters c{counters};
+  const double theta2 = theta * theta;
+  if (precision == 0) {
+    WalkArgs<F64> a{t.rec64, pos64, 0, (uint32_t)leaf, theta2, ax, ay, az, c};
+    launch_forest(a, count, ws, f.sys, tasks, ntasks, queue, cus, s);
+  } else if (precision == 1) {
+    WalkArgs<F32> a{t.rec32, pos32, 0, (uint32_t)leaf, theta2, ax, ay, az, c};
+    launch_forest(a, count, ws, f.sys, tasks, ntasks, queue, cus, s);
+  } else {
+    WalkArgs<D64> a{t.recd, pos64, 0, (uint32_t)leaf, theta2, ax, ay, az, c};
+    launch_forest(a, count, ws, f.sys, tasks, ntasks, queue, cus, s);
   }
   return hipGetLastError();
 }

@@ -177,13 +177,18 @@ class Simulation:
         return np.stack(a[:3], axis=1), np.stack(a[3:], axis=1)
 
     def export_tree(self, layout: int = KD_LAYOUT_COMPACT) -> Tree:
+        return _export(lambda v: self._L.kd_export_tree(self._ctx, layout, v), self.n)
+
+
+def _export(call, n: int) -> Tree:
+    """Run an export entry point twice: once for the length, once to fill."""
+    if True:
         v = KdTreeView()
         v.n_nodes = 0
-        rc = self._L.kd_export_tree(self._ctx, layout, C.byref(v))
+        rc = call(C.byref(v))
         if rc != _lib.KD_ERR_CAPACITY:
             check(rc)
         nn = v.n_nodes
-        n = self.n
         a = dict(
             is_leaf=np.zeros(nn, np.int32), split_dim=np.zeros(nn, np.int32),
             num_parts=np.zeros(nn, np.int64), first=np.zeros(nn, np.int64),
@@ -198,11 +203,93 @@ class Simulation:
             setattr(v, k, iptr(a[k]))
         for k in ("split_val", "m", "cmx", "cmy", "cmz", "size"):
             setattr(v, k, dptr(a[k]))
-        check(self._L.kd_export_tree(self._ctx, layout, C.byref(v)))
+        check(call(C.byref(v)))
         return Tree(a["is_leaf"], a["split_dim"], a["num_parts"], a["first"],
                     a["split_val"], a["m"],
                     np.stack([a["cmx"], a["cmy"], a["cmz"]], axis=1), a["size"],
                     a["left"], a["right"], a["perm"])
+
+
+class Batch:
+    """Many independent systems in one context (kd_batch): built and stepped
+    together.  Bodies are concatenated in system order."""
+
+    def __init__(self, sizes, theta: float = 0.3, dt: float = 1e-3, leaf_size: int = 7,
+                 precision: int = KD_PREC_MIXED, count_interactions: bool = False):
+        L = _lib.lib()
+        self._L = L
+        self.sizes = np.ascontiguousarray(sizes, np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
+        p = _lib.default_params(theta=theta, dt=dt, leaf_size=leaf_size, precision=precision,
+                                count_interactions=int(count_interactions))
+        b = C.c_void_p()
+        check(L.kd_batch_create(len(self.sizes), iptr(self.sizes), C.byref(p), C.byref(b)))
+        self._b = b
+        self.n = int(self.offsets[-1])
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            self._L.kd_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, bodies: dict):
+        cols = [np.ascontiguousarray(bodies[f], np.float64) for f in FIELDS]
+        assert all(len(c) == self.n for c in cols)
+        check(self._L.kd_batch_upload(self._b, *[dptr(c) for c in cols]))
+
+    def download(self) -> dict:
+        out = {f: np.zeros(self.n) for f in FIELDS[:6]}
+        check(self._L.kd_batch_download(self._b, *[dptr(out[f]) for f in FIELDS[:6]]))
+        return out
+
+    def accel(self) -> np.ndarray:
+        a = [np.zeros(self.n) for _ in range(3)]
+        check(self._L.kd_batch_accel(self._b, *[dptr(x) for x in a]))
+        return np.stack(a, axis=1)
+
+    def step(self, steps: int = 1):
+        check(self._L.kd_batch_step(self._b, steps))
+
+    def export_tree(self, sys: int, layout: int = KD_LAYOUT_COMPACT) -> Tree:
+        return _export(lambda v: self._L.kd_batch_export_tree(self._b, sys, layout, v),
+                       int(self.sizes[sys]))
+
+    def set_tuning(self, key: str, value: int):
+        check(self._L.kd_batch_set_tuning(self._b, key.encode(), int(value)))
+
+    def autotune(self, reps: int = 3) -> dict:
+        out = np.zeros(3)
+        check(self._L.kd_batch_autotune(self._b, reps, dptr(out)))
+        return {"walk_threads": int(out[0]), "walk_waves_per_cu": int(out[1]),
+                "walk_ms": float(out[2])}
+
+    def counters(self) -> np.ndarray:
+        out = np.zeros(5, np.int64)
+        check(self._L.kd_batch_counters(self._b, iptr(out)))
+        return out
+
+    def system(self, arrays: dict, i: int) -> dict:
+        """The slice of system i from concatenated arrays."""
+        lo, hi = int(self.offsets[i]), int(self.offsets[i + 1])
+        return {k: v[lo:hi] for k, v in arrays.items()}
+
+
+def ensemble(sizes, seed0: int = 0) -> dict:
+    """Concatenated ring systems, system i from seed seed0 + i."""
+    parts = [ring(int(n), seed=seed0 + i) for i, n in enumerate(sizes)]
+    return {f: np.concatenate([p[f] for p in parts]) for f in FIELDS}
 
 
 def allocate_node_vec(n: int, leaf_size: int = 7, layout: int = KD_LAYOUT_COMPACT) -> int:
@@ -277,7 +364,7 @@ def ring(n: int, seed: int = 0) -> dict:
     return _lib.ic("ring", n, seed)
 
 
-__all__ = ["Simulation", "Tree", "allocate_node_vec", "simple_sim", "build_tree",
+__all__ = ["Simulation", "Batch", "ensemble", "Tree", "allocate_node_vec", "simple_sim", "build_tree",
            "calc_accel", "calc_accel_tree", "circular_orbits", "two_bodies", "ring",
            "KD_PREC_FP64", "KD_PREC_MIXED", "KD_PREC_MIXED_D64",
            "KD_LAYOUT_COMPACT", "KD_LAYOUT_PADDED"]
