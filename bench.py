@@ -110,6 +110,53 @@ def cpu_baseline(bodies, n, sample, threads, dt):
                 sample=what + f"; one step extrapolated = {step_s:.2f} s")
 
 
+def batch_sizes(nsys, seed):
+    """Per-system N, log-uniform in [2000, 200000]."""
+    rng = np.random.default_rng(seed)
+    return np.floor(np.exp(rng.uniform(np.log(2000), np.log(200000), nsys))).astype(np.int64)
+
+
+def run_batch(nsys, steps, loop_steps, theta, leaf, dt, seed):
+    """The ensemble: nsys ring systems (system i from seed seed + 1 + i) built
+    and stepped together by one graph-replayed batch step, against a loop
+    that steps each system in its own context (kd_step per system)."""
+    sizes = batch_sizes(nsys, seed)
+    total = int(sizes.sum())
+    bodies = K.ensemble(sizes, seed0=seed + 1)
+    with K.Batch(sizes, theta=theta, dt=dt, leaf_size=leaf, precision=K.KD_PREC_MIXED) as b:
+        b.upload(bodies)
+        tune = b.autotune(reps=2)
+        log(f"[bench] batch: autotuned walk {tune}")
+        b.step(1)  # captures the two step graphs
+        t0 = time.perf_counter()
+        b.step(steps)
+        t1 = time.perf_counter()
+    rate = total * steps / (t1 - t0)
+    sims = []
+    try:
+        for i, nn in enumerate(sizes):
+            s = K.Simulation(int(nn), theta=theta, dt=dt, leaf_size=leaf,
+                             precision=K.KD_PREC_MIXED)
+            s.upload({k: v[b.offsets[i]:b.offsets[i + 1]] for k, v in bodies.items()})
+            sims.append(s)
+        for s in sims:
+            s.step(1)  # warm-up
+        t2 = time.perf_counter()
+        for s in sims:
+            s.step(loop_steps)
+        t3 = time.perf_counter()
+    finally:
+        for s in sims:
+            s.close()
+    loop_rate = total * loop_steps / (t3 - t2)
+    return dict(systems=nsys, bodies=total, n_min=int(sizes.min()), n_max=int(sizes.max()),
+                steps=steps, value=rate, ms_per_step=1e3 * (t1 - t0) / steps,
+                per_system_loop=dict(value=loop_rate, steps=loop_steps,
+                                     ms_per_step=1e3 * (t3 - t2) / loop_steps),
+                speedup_vs_loop=rate / loop_rate, walk_shape=tune,
+                unit="particle-steps/s", precision="mixed", graph=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,7 +170,11 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp64", "both"], default="both")
+    ap.add_argument("--only", choices=["mixed", "fp64", "both", "batch"], default="both")
+    ap.add_argument("--batch-systems", type=int, default=512)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-loop-steps", type=int, default=2)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -131,6 +182,12 @@ def main():
         sys.exit(2)
 
     n, dt = args.n, 1e-3
+    if args.only == "batch":
+        log(f"[bench] batch of {args.batch_systems} systems, {args.batch_steps} steps")
+        rb = run_batch(args.batch_systems, args.batch_steps, args.batch_loop_steps, args.theta,
+                       args.leaf, dt, args.seed)
+        print(json.dumps(rb), flush=True)
+        return
     log(f"[bench] generating ring ICs n={n} seed={args.seed}")
     bodies = K.ring(n, seed=args.seed)
     res = {}
@@ -177,6 +234,13 @@ def main():
         cpu = cpu_baseline(bodies, n, args.cpu_sample, threads, dt)
         log(f"[bench] cpu {cpu}")
 
+    batch = None
+    if not args.no_batch:
+        log(f"[bench] batch of {args.batch_systems} systems, {args.batch_steps} steps")
+        batch = run_batch(args.batch_systems, args.batch_steps, args.batch_loop_steps,
+                          args.theta, args.leaf, dt, args.seed)
+        log(f"[bench] batch {json.dumps(batch)}")
+
     out = {
         "metric": "particle-steps/s @N=10M theta=0.3: FP32-mixed vs FP64 GPU @ median force err<=1e-6",
         "value": r["rate"],
@@ -198,6 +262,7 @@ def main():
         "cpu_baseline": cpu,
         "phases_ms": {k: round(r[k], 3) for k in ("build_ms", "traverse_ms", "kick_drift_ms")},
         "force_error": err,
+        "batch": batch,
     }
     if "fp64" in res and "mixed" in res:
         out["fp64_gpu"] = {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms_per_step"],

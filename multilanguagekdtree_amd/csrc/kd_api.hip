@@ -76,7 +76,7 @@ struct kd_ctx {
   Forest forest;
   std::vector<SysDesc> hsys;
   SysDesc *dsys = nullptr;
-  uint32_t *seg_lo = nullptr, *seg_hi = nullptr;
+  uint32_t *sob = nullptr;  // system of every 64 body positions
 };
 
 template <typename T>
@@ -140,22 +140,19 @@ static int install_forest(kd_ctx *c) {
   f = Forest();
   f.nsys = nsys;
   f.sys = c->dsys;
-  f.seg_lo = c->seg_lo;
-  f.seg_hi = c->seg_hi;
-  std::vector<uint32_t> lo(nsys), hi(nsys);
+  f.sys_of_block = c->sob;
+  std::vector<uint32_t> sob;
   for (int i = 0; i < nsys; ++i) {
     const SysDesc &d = c->hsys[i];
     f.max_sub_level = std::max(f.max_sub_level, (int)d.sub_level);
     f.max_n = std::max(f.max_n, d.n);
     f.max_slots = std::max(f.max_slots, d.slots);
     f.total = std::max(f.total, d.body_off + d.n);
-    lo[i] = d.body_off;
-    hi[i] = d.body_off + d.n;
+    sob.resize((d.body_off + d.n + 63) / 64, (uint32_t)i);
   }
   HIP_TRY(hipMemcpyAsync(c->dsys, c->hsys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice,
                          c->stream));
-  HIP_TRY(hipMemcpyAsync(c->seg_lo, lo.data(), 4 * nsys, hipMemcpyHostToDevice, c->stream));
-  HIP_TRY(hipMemcpyAsync(c->seg_hi, hi.data(), 4 * nsys, hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(c->sob, sob.data(), 4 * sob.size(), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(launch_shape(c->tree, f, c->params.leaf_size, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));  // the host tables above are temporaries
   return KD_OK;
@@ -166,7 +163,7 @@ static int ensure_nodes(kd_ctx *c) {
   c->shape = sh;
   int rc = alloc_nodes(c, sh.slots);
   if (rc) return rc;
-  c->hsys.assign(1, make_sys(sh, 0, 0));
+  c->hsys.assign(1, make_sys(sh, 0, 0, 0));
   return install_forest(c);
 }
 
@@ -263,8 +260,7 @@ static int create_impl(kd_ctx *c, int64_t cap, const Sizing &z) {
   c->b.sort_tmp_bytes = z.sort_bytes;
   ALLOC(c->allocs, *(uint8_t **)&c->b.sort_tmp, c->b.sort_tmp_bytes);
   ALLOC(c->allocs, c->dsys, z.nsys);
-  ALLOC(c->allocs, c->seg_lo, z.nsys);
-  ALLOC(c->allocs, c->seg_hi, z.nsys);
+  ALLOC(c->allocs, c->sob, N / 64 + 1);
   ALLOC(c->allocs, c->counters, 8);
   ALLOC(c->allocs, c->bad, 4);
   return KD_OK;
@@ -1004,8 +1000,9 @@ static int batch_step_enqueue(kd_batch *b) {
   return KD_OK;
 }
 
-// Capture the step that starts from state buffer `from` into a graph.  A
-// captured memset node is refused: the step clears memory with kernels only.
+// Capture the step that starts from state buffer `from` into a graph.  The
+// sort library records memset nodes; each becomes a fill kernel node, and a
+// graph that still holds one is refused.
 static int batch_capture(kd_batch *b, int from) {
   kd_ctx *c = b->c;
   const int saved = c->cur;
@@ -1020,6 +1017,12 @@ static int batch_capture(kd_batch *b, int from) {
     return rc;
   }
   if (e != hipSuccess) return fail(KD_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+  int replaced = 0;
+  e = graph_replace_memsets(g, &replaced);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(KD_ERR_HIP, "replacing memset nodes: %s", hipGetErrorString(e));
+  }
   size_t nn = 0;
   e = hipGraphGetNodes(g, nullptr, &nn);
   std::vector<hipGraphNode_t> nodes(nn);
@@ -1074,7 +1077,7 @@ extern "C" int kd_batch_create(int64_t nsys, const int64_t *n, const kd_params *
   for (int64_t i = 0; i < nsys; ++i) {
     if (n[i] < 1 || n[i] > 0x7fffffffLL) return fail(KD_ERR_ARG, "n[%lld] out of range", (long long)i);
     const Shape sh = make_shape(n[i], p.leaf_size);
-    hs[i] = make_sys(sh, (uint32_t)body, (uint32_t)slots);
+    hs[i] = make_sys(sh, (uint32_t)body, (uint32_t)slots, (uint32_t)total_n);
     body += ((uint64_t)n[i] + 63) & ~63ull;
     slots += (uint64_t)sh.slots;
     total_n += n[i];

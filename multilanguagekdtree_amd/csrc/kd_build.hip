@@ -18,7 +18,6 @@
 // Ties on a split coordinate are resolved by original index, lazily: only a
 // run of equal keys that straddles the median is reordered.
 #include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_segmented_radix_sort.hpp>
 
 #include <algorithm>
 
@@ -177,6 +176,19 @@ __global__ void k_make_keys(const double *__restrict__ p, uint64_t *__restrict__
   ids[j] = j;
 }
 
+// Forest: system of every id of a coordinate-sorted list (nsys for padding
+// positions), the key of the stable sort that groups the list by system.
+__global__ void k_sys_keys(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ sob,
+                           const SysDesc *__restrict__ sys, uint32_t nsys,
+                           uint32_t *__restrict__ key, uint32_t n) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t id = ids[j];
+  const uint32_t q = sob[id >> 6];
+  const SysDesc S = sys[q];
+  key[j] = (id - S.body_off < S.n) ? q : nsys;
+}
+
 // ---------------------------------------------------------------------------
 // device-wide levels
 // ---------------------------------------------------------------------------
@@ -197,7 +209,7 @@ __global__ void k_level_split(Lists L, Coords P, TreeDev t0, const SysDesc *sys,
   const TreeDev t = tree_at(t0, S.node_off);
   tieflag += blockIdx.y * tf_stride;
   uint32_t h = nseg - 1 + k;
-  uint32_t s = t.seg_start[h] + S.body_off, c = t.seg_count[h];
+  uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
   double ext[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
@@ -234,7 +246,7 @@ __global__ __launch_bounds__(1024) void k_level_tiefix(Lists L, Lists scratch,
   if (level >= S.sub_level || !tieflag[blockIdx.y * tf_stride + k]) return;
   const TreeDev t = tree_at(t0, S.node_off);
   uint32_t h = (1u << level) - 1 + k;
-  uint32_t s = t.seg_start[h] + S.body_off, c = t.seg_count[h];
+  uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
   uint32_t *ls = L.l[sd];
   uint32_t *tmp = scratch.l[sd];
@@ -319,7 +331,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_flags(Lists L, TreeDev t0
   const TreeDev t = tree_at(t0, S.node_off);
   uint32_t k = blockIdx.x / T, q = blockIdx.x % T;
   uint32_t h = (1u << level) - 1 + k;
-  uint32_t s = t.seg_start[h] + S.body_off, c = t.seg_count[h];
+  uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
   uint32_t mid = s + c / 2, e = s + c;
   const uint32_t *ls = L.l[sd];
@@ -344,7 +356,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_count(Lists L, TreeDev t0
   cnt += (size_t)blockIdx.y * 3 * nseg * T;
   uint32_t k = blockIdx.x / T, q = blockIdx.x % T;
   uint32_t h = nseg - 1 + k;
-  uint32_t s = t.seg_start[h] + S.body_off, c = t.seg_count[h];
+  uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
   uint32_t e = s + c;
   uint32_t base = s + q * LVL_TILE;
@@ -396,7 +408,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(Lists L, Lists Lo
   cnt += (size_t)blockIdx.y * 3 * nseg * T;
   uint32_t k = blockIdx.x / T, q = blockIdx.x % T;
   uint32_t h = nseg - 1 + k;
-  uint32_t s = t.seg_start[h] + S.body_off, c = t.seg_count[h];
+  uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
   uint32_t mid = s + c / 2, e = s + c;
   uint32_t base = s + q * LVL_TILE + threadIdx.x * LVL_ITEMS;
@@ -552,15 +564,16 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   const TreeDev t = tree_at(a.t, S.node_off);
   const Lists L = a.L[level & 1];
   const uint32_t h0 = (1u << level) - 1 + k;
-  // s0l: first body of the segment within its system; s0: in the body arrays
+  // s0l: first body of the segment within its system; s0: in the body
+  // arrays (tree order); sl: in the sorted lists
   const uint32_t s0l = t.seg_start[h0], n0 = t.seg_count[h0];
-  const uint32_t s0 = s0l + S.body_off;
+  const uint32_t s0 = s0l + S.body_off, sl = s0l + S.list_off;
   if (n0 == 0) return;
   const int leaf = a.leaf;
   const int tid = threadIdx.x;
 
   for (uint32_t i = tid; i < n0; i += NT) {
-    uint32_t id = L.l[0][s0 + i];
+    uint32_t id = L.l[0][sl + i];
     ids[i] = id;
     lst[0][0][i] = (uint16_t)i;
     a.loc[id] = i;
@@ -568,8 +581,8 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   __threadfence_block();
   __syncthreads();
   for (uint32_t i = tid; i < n0; i += NT) {
-    lst[0][1][i] = (uint16_t)a.loc[L.l[1][s0 + i]];
-    lst[0][2][i] = (uint16_t)a.loc[L.l[2][s0 + i]];
+    lst[0][1][i] = (uint16_t)a.loc[L.l[1][sl + i]];
+    lst[0][2][i] = (uint16_t)a.loc[L.l[2][sl + i]];
   }
   __syncthreads();
 
@@ -883,9 +896,10 @@ Shape make_shape(int64_t n, int leaf) {
 }
 
 
-SysDesc make_sys(const Shape &sh, uint32_t body_off, uint32_t node_off) {
+SysDesc make_sys(const Shape &sh, uint32_t body_off, uint32_t node_off, uint32_t list_off) {
   SysDesc d{};
   d.body_off = body_off;
+  d.list_off = list_off;
   d.n = (uint32_t)sh.n;
   d.node_off = node_off;
   d.slots = (uint32_t)sh.slots;
@@ -894,16 +908,22 @@ SysDesc make_sys(const Shape &sh, uint32_t body_off, uint32_t node_off) {
   return d;
 }
 
+static int bits_for(uint32_t v) {
+  int b = 1;
+  while (b < 32 && (v >> b) != 0) ++b;
+  return b;
+}
+
 size_t sort_temp_bytes(uint32_t n, int nsys) {
   size_t bytes = 0;
-  if (nsys <= 1) {
-    (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
-                                    (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 64);
-  } else {
-    (void)rocprim::segmented_radix_sort_pairs(
-        nullptr, bytes, (uint64_t *)nullptr, (uint64_t *)nullptr, (uint32_t *)nullptr,
-        (uint32_t *)nullptr, n, (unsigned)nsys, (const uint32_t *)nullptr,
-        (const uint32_t *)nullptr, 0, 64);
+  (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                  (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 64);
+  if (nsys > 1) {
+    size_t b2 = 0;
+    (void)rocprim::radix_sort_pairs(nullptr, b2, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                    (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0,
+                                    bits_for((uint32_t)nsys));
+    bytes = std::max(bytes, b2);
   }
   return bytes;
 }
@@ -937,21 +957,28 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   const double *coord[3] = {in.x, in.y, in.z};
   const unsigned ny = (unsigned)f.nsys;
   hipError_t e;
-  // a single system sorts its n bodies (at offset 0); a forest sorts every
-  // system's range as its own segment
+  // a single system sorts its n bodies; a forest sorts every body position
+  // by coordinate, then stably by system, which leaves each system's bodies
+  // in coordinate order at [list_off, list_off + n) (padding last)
   const uint32_t nsort = f.nsys == 1 ? f.max_n : f.total;
   if (sort_temp_bytes(nsort, f.nsys) > b.sort_tmp_bytes) return hipErrorInvalidValue;
   for (int d = 0; d < 3; ++d) {
     k_make_keys<<<(nsort + 255) / 256, 256, 0, s>>>(coord[d], b.keys_in, b.ids_in, nsort);
     size_t bytes = b.sort_tmp_bytes;
-    if (f.nsys == 1)
-      e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, b.keys_in, b.keys_out, b.ids_in, b.L[d],
-                                    nsort, 0, 64, s);
-    else
-      e = rocprim::segmented_radix_sort_pairs(b.sort_tmp, bytes, b.keys_in, b.keys_out, b.ids_in,
-                                              b.L[d], nsort, (unsigned)f.nsys, f.seg_lo,
-                                              f.seg_hi, 0, 64, s);
+    uint32_t *by_key = f.nsys == 1 ? b.L[d] : b.Lb[d];
+    e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, b.keys_in, b.keys_out, b.ids_in, by_key,
+                                  nsort, 0, 64, s);
     if (e != hipSuccess) return e;
+    if (f.nsys > 1) {
+      uint32_t *skey = reinterpret_cast<uint32_t *>(b.keys_in);
+      uint32_t *skey_out = reinterpret_cast<uint32_t *>(b.keys_out);
+      k_sys_keys<<<(nsort + 255) / 256, 256, 0, s>>>(by_key, f.sys_of_block, f.sys,
+                                                     (uint32_t)f.nsys, skey, nsort);
+      bytes = b.sort_tmp_bytes;
+      e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, skey, skey_out, by_key, b.L[d], nsort, 0,
+                                    bits_for((uint32_t)f.nsys), s);
+      if (e != hipSuccess) return e;
+    }
   }
   Lists L{{b.L[0], b.L[1], b.L[2]}};
   Lists Lb{{b.Lb[0], b.Lb[1], b.Lb[2]}};

@@ -89,10 +89,11 @@ __host__ __device__ inline TreeDev tree_at(const TreeDev &t, uint32_t o) {
 
 // One system of a forest: independent trees built and stepped together.
 // Body arrays hold every system's bodies at [body_off, body_off + n) (body_off
-// a multiple of 64, so no wavefront spans two systems); node arrays hold its
-// heap at [node_off, node_off + slots).  Node records store body positions
-// relative to body_off.  A single system is a forest of one with both
-// offsets 0.
+// a multiple of 64, so no wavefront spans two systems); the sorted id lists
+// hold them at [list_off, list_off + n) (systems back to back); node arrays
+// hold its heap at [node_off, node_off + slots).  Node records store body
+// positions relative to body_off.  A single system is a forest of one with
+// every offset 0.
 struct SysDesc {
   uint32_t body_off;
   uint32_t n;
@@ -100,7 +101,8 @@ struct SysDesc {
   uint32_t slots;
   uint32_t sub_level;  // levels split device-wide; the subtree builder takes the next
   uint32_t levels;
-  uint32_t pad0, pad1;
+  uint32_t list_off;
+  uint32_t pad0;
 };
 
 struct Forest {
@@ -110,7 +112,7 @@ struct Forest {
   uint32_t max_n = 0;
   uint32_t max_slots = 0;
   uint32_t total = 0;            // body positions, padding included
-  const uint32_t *seg_lo = nullptr, *seg_hi = nullptr;  // device, sort segment bounds
+  const uint32_t *sys_of_block = nullptr;  // device: system of body positions [64i, 64i+64)
 };
 
 // Body state in "working order" (the previous step's tree order).
@@ -141,7 +143,7 @@ struct BuildBufs {
 };
 
 Shape make_shape(int64_t n, int leaf);
-SysDesc make_sys(const Shape &sh, uint32_t body_off, uint32_t node_off);
+SysDesc make_sys(const Shape &sh, uint32_t body_off, uint32_t node_off, uint32_t list_off);
 size_t sort_temp_bytes(uint32_t n, int nsys);
 // per-tile count entries and tie flags the device-wide levels of a forest need
 size_t level_cnt_entries(const Forest &f);
@@ -166,6 +168,9 @@ hipError_t launch_traverse_forest(int precision, bool count, const WalkShape &ws
                                   uint32_t *queue, int cus, double theta, int leaf, double *ax,
                                   double *ay, double *az, unsigned long long *counters,
                                   hipStream_t s);
+// Replace every memset node of a captured graph by a fill kernel node with
+// the same dependencies (memset nodes are not replayed reliably here).
+hipError_t graph_replace_memsets(hipGraph_t g, int *replaced);
 hipError_t launch_kick_drift(const StateDev &s, const double *ax, const double *ay,
                              const double *az, uint32_t n, double dt, int *bad,
                              hipStream_t st);

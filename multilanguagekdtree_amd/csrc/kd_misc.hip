@@ -1,4 +1,7 @@
 // kd_misc.hip -- integrator, reordering and checking kernels.
+#include <algorithm>
+#include <vector>
+
 #include "kd_internal.hpp"
 
 namespace kd {
@@ -142,6 +145,71 @@ hipError_t launch_direct(const StateDev &s, uint32_t n, const uint32_t *wpos, ui
                          double *out, hipStream_t st) {
   k_direct<<<k, 256, 0, st>>>(s.x, s.y, s.z, s.m, n, wpos, out);
   return hipGetLastError();
+}
+
+}  // namespace kd
+
+namespace kd {
+
+// Fill of a 2D region (the work of a memset node): width elements of
+// elem_size bytes per row, height rows pitch bytes apart.
+__global__ void k_fill(char *dst, unsigned elem_size, size_t width, size_t height, size_t pitch,
+                       unsigned value) {
+  const size_t total = width * height;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    char *p = dst + (i / width) * pitch + (i % width) * elem_size;
+    if (elem_size == 1)
+      *reinterpret_cast<uint8_t *>(p) = (uint8_t)value;
+    else if (elem_size == 2)
+      *reinterpret_cast<uint16_t *>(p) = (uint16_t)value;
+    else
+      *reinterpret_cast<uint32_t *>(p) = value;
+  }
+}
+
+hipError_t graph_replace_memsets(hipGraph_t g, int *replaced) {
+  *replaced = 0;
+  size_t nn = 0;
+  hipError_t e = hipGraphGetNodes(g, nullptr, &nn);
+  if (e != hipSuccess) return e;
+  std::vector<hipGraphNode_t> nodes(nn);
+  if (nn && (e = hipGraphGetNodes(g, nodes.data(), &nn)) != hipSuccess) return e;
+  for (hipGraphNode_t m : nodes) {
+    hipGraphNodeType ty;
+    if ((e = hipGraphNodeGetType(m, &ty)) != hipSuccess) return e;
+    if (ty != hipGraphNodeTypeMemset) continue;
+    hipMemsetParams mp;
+    if ((e = hipGraphMemsetNodeGetParams(m, &mp)) != hipSuccess) return e;
+    if (mp.elementSize != 1 && mp.elementSize != 2 && mp.elementSize != 4)
+      return hipErrorInvalidValue;
+    size_t nd = 0, nt = 0;
+    if ((e = hipGraphNodeGetDependencies(m, nullptr, &nd)) != hipSuccess) return e;
+    if ((e = hipGraphNodeGetDependentNodes(m, nullptr, &nt)) != hipSuccess) return e;
+    std::vector<hipGraphNode_t> deps(nd), dents(nt);
+    if (nd && (e = hipGraphNodeGetDependencies(m, deps.data(), &nd)) != hipSuccess) return e;
+    if (nt && (e = hipGraphNodeGetDependentNodes(m, dents.data(), &nt)) != hipSuccess) return e;
+    char *dst = static_cast<char *>(mp.dst);
+    unsigned elem = mp.elementSize;
+    size_t width = mp.width, height = mp.height ? mp.height : 1, pitch = mp.pitch;
+    unsigned value = mp.value;
+    void *args[] = {&dst, &elem, &width, &height, &pitch, &value};
+    const size_t total = width * height;
+    hipKernelNodeParams kp{};
+    kp.func = reinterpret_cast<void *>(&k_fill);
+    kp.blockDim = dim3(256);
+    kp.gridDim = dim3((unsigned)std::min<size_t>((total + 255) / 256 + 1, 4096));
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    kp.sharedMemBytes = 0;
+    hipGraphNode_t k = nullptr;
+    if ((e = hipGraphAddKernelNode(&k, g, deps.data(), nd, &kp)) != hipSuccess) return e;
+    for (hipGraphNode_t d : dents)
+      if ((e = hipGraphAddDependencies(g, &k, &d, 1)) != hipSuccess) return e;
+    if ((e = hipGraphDestroyNode(m)) != hipSuccess) return e;
+    ++*replaced;
+  }
+  return hipSuccess;
 }
 
 }  // namespace kd

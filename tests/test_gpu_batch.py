@@ -141,3 +141,20 @@ def test_batch_errors(bodies):
         with pytest.raises(_lib.KdError) as e:
             bt.upload(bad)
         assert e.value.code == _lib.KD_ERR_NONFINITE
+
+
+def test_graph_replays_equal_eager_steps_large():
+    # large enough that the sort library takes its multi-pass path (whose
+    # memset nodes the capture turns into fill kernels)
+    sizes = np.random.default_rng(5).integers(2000, 200000, size=48)
+    bodies = K.ensemble(sizes, seed0=500)
+    with K.Batch(sizes) as g, K.Batch(sizes) as e:
+        g.upload(bodies)
+        e.upload(bodies)
+        e.set_tuning("graph", 0)
+        for k in range(3):
+            g.step(1)
+            e.step(1)
+            sg, se = g.download(), e.download()
+            for f in ("x", "vx", "z"):
+                np.testing.assert_array_equal(sg[f], se[f], err_msg=f"replay {k} {f}")
