@@ -210,6 +210,34 @@ int kd_batch_autotune(kd_batch *b, int64_t reps, double out[3]);
  * since the last kd_batch_accel / kd_batch_step began. */
 int kd_batch_counters(kd_batch *b, int64_t out[5]);
 
+/* ---- differentiation ----
+ * steps of simple_sim (Parallel/RustVersion/src/array_kd_tree.rs:623-664)
+ * with the tree topology frozen within each step (the step's tree and every
+ * body's accept/open decisions are fixed by the forward run), and the
+ * reverse sweep of that function: gradients of a loss of the final state
+ * with respect to the initial positions, initial velocities and masses.
+ * The forward keeps a checkpoint of the state every `every` steps; the
+ * reverse sweep recomputes each segment's trees from its checkpoint.
+ * precision KD_PREC_FP64 or KD_PREC_MIXED (terms of the reverse sweep in the
+ * forward's arithmetic, adjoint sums in FP64).  All arrays are DEVICE
+ * pointers in original body order.  No counterpart in the reference. */
+typedef struct kd_adj kd_adj;
+int kd_adj_create(int64_t n, int64_t steps, int64_t every, const kd_params *params,
+                  kd_adj **out);
+int kd_adj_destroy(kd_adj *a);
+/* Run the steps from (x0..vz0, m); the final state goes to x..vz (any may
+ * be NULL). */
+int kd_adj_forward(kd_adj *a, const double *x0, const double *y0, const double *z0,
+                   const double *vx0, const double *vy0, const double *vz0,
+                   const double *m, double *x, double *y, double *z, double *vx,
+                   double *vy, double *vz);
+/* Given dL/dp_final (gp*) and dL/dv_final (gv*; NULL = 0), the gradients
+ * dL/dp0, dL/dv0 and dL/dm (any output may be NULL). */
+int kd_adj_backward(kd_adj *a, const double *gpx, const double *gpy, const double *gpz,
+                    const double *gvx, const double *gvy, const double *gvz,
+                    double *gx0, double *gy0, double *gz0, double *gvx0,
+                    double *gvy0, double *gvz0, double *gm);
+
 /* Synthetic initial conditions (host only, no device needed).
  * kd_ic_ring: body 0 mass 1 at rest at the origin; bodies 1..n-1 mass 1e-14,
  * r ~ U[1,2), phi ~ U[0,2pi), z ~ N(0, 1e-3), v = sqrt(1/r)(-sin phi, cos phi, 0).

@@ -104,6 +104,11 @@ SIGNATURES = {
     "kd_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
     "kd_batch_autotune": (C.c_int, [C.c_void_p, C.c_int64, _d]),
     "kd_batch_counters": (C.c_int, [C.c_void_p, _i64]),
+    "kd_adj_create": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, _P(KdParams),
+                                _P(C.c_void_p)]),
+    "kd_adj_destroy": (C.c_int, [C.c_void_p]),
+    "kd_adj_forward": (C.c_int, [C.c_void_p] + [C.c_void_p] * 13),
+    "kd_adj_backward": (C.c_int, [C.c_void_p] + [C.c_void_p] * 13),
     "kd_ic_ring": (C.c_int, [C.c_uint64, C.c_int64] + [_d] * 7),
     "kd_ic_circular_orbits": (C.c_int, [C.c_uint64, C.c_int64] + [_d] * 7),
     "kd_ic_two_bodies": (C.c_int, [_d] * 7),

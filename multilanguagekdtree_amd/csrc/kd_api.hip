@@ -1341,3 +1341,224 @@ extern "C" int kd_batch_counters(kd_batch *b, int64_t out[5]) {
   if (!b) return fail(KD_ERR_ARG, "batch is NULL");
   return kd_counters(b->c, out);
 }
+
+// ---------------------------------------------------------------------------
+// differentiation: forward with checkpoints, reverse sweep with recompute
+// ---------------------------------------------------------------------------
+
+struct kd_adj {
+  kd_ctx *c = nullptr;
+  int64_t n = 0, steps = 0, every = 0, nck = 0;
+  double *ck = nullptr;        // nck checkpoints x 7 arrays x n (working order)
+  uint32_t *ck_oid = nullptr;  // nck x n
+  // tape of one segment (every steps)
+  double4 *t_pos64 = nullptr;
+  float4 *t_pos32 = nullptr;
+  uint32_t *t_oid = nullptr;
+  NodeRec64 *t_rec64 = nullptr;
+  NodeRec32 *t_rec32 = nullptr;
+  int64_t slots = 0;
+  double *lp[3] = {nullptr, nullptr, nullptr}, *lv[3] = {nullptr, nullptr, nullptr},
+         *u[3] = {nullptr, nullptr, nullptr};
+  double *lm = nullptr, *gnode = nullptr;
+  std::vector<void *> allocs;
+  bool forward_done = false;
+};
+
+extern "C" int kd_adj_destroy(kd_adj *a) {
+  if (!a) return KD_OK;
+  if (a->c) {
+    (void)hipSetDevice(a->c->device);
+    (void)hipStreamSynchronize(a->c->stream);
+  }
+  for (void *p : a->allocs) (void)hipFree(p);
+  if (a->c) kd_destroy(a->c);
+  delete a;
+  return KD_OK;
+}
+
+template <typename T>
+static int adj_alloc(kd_adj *a, T **p, size_t count) {
+  void *q = nullptr;
+  hipError_t e = hipMalloc(&q, count ? count * sizeof(T) : 256);
+  if (e != hipSuccess) return fail(KD_ERR_NOMEM, "hipMalloc failed: %s", hipGetErrorString(e));
+  a->allocs.push_back(q);
+  *p = (T *)q;
+  return KD_OK;
+}
+
+extern "C" int kd_adj_create(int64_t n, int64_t steps, int64_t every, const kd_params *params,
+                             kd_adj **out) {
+  if (!out) return fail(KD_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (n < 1 || n > 0x7fffffffLL) return fail(KD_ERR_ARG, "n out of range");
+  if (steps < 1 || every < 1) return fail(KD_ERR_ARG, "steps and checkpoint interval must be >= 1");
+  if (params && params->precision == KD_PREC_MIXED_D64)
+    return fail(KD_ERR_ARG, "the reverse sweep supports KD_PREC_FP64 and KD_PREC_MIXED");
+  kd_adj *a = new kd_adj();
+  int rc = kd_create(n, params, &a->c);
+  if (rc) {
+    std::string msg = g_err;
+    delete a;
+    g_err = msg;
+    return rc;
+  }
+  a->n = n;
+  a->steps = steps;
+  a->every = std::min(every, steps);
+  a->nck = (steps + a->every - 1) / a->every;
+  a->slots = make_shape(n, a->c->params.leaf_size).slots;
+  const size_t N = (size_t)n, E = (size_t)a->every, S = (size_t)a->slots;
+  auto bail = [&](int code) {
+    std::string msg = g_err;
+    kd_adj_destroy(a);
+    g_err = msg;
+    return code;
+  };
+  if ((rc = adj_alloc(a, &a->ck, (size_t)a->nck * 7 * N)) ||
+      (rc = adj_alloc(a, &a->ck_oid, (size_t)a->nck * N)) ||
+      (rc = adj_alloc(a, &a->t_pos64, E * N)) || (rc = adj_alloc(a, &a->t_pos32, E * N)) ||
+      (rc = adj_alloc(a, &a->t_oid, E * N)) || (rc = adj_alloc(a, &a->t_rec64, E * S)) ||
+      (rc = adj_alloc(a, &a->t_rec32, E * S)) || (rc = adj_alloc(a, &a->lm, N)) ||
+      (rc = adj_alloc(a, &a->gnode, 4 * S)))
+    return bail(rc);
+  for (int d = 0; d < 3; ++d)
+    if ((rc = adj_alloc(a, &a->lp[d], N)) || (rc = adj_alloc(a, &a->lv[d], N)) ||
+        (rc = adj_alloc(a, &a->u[d], N)))
+      return bail(rc);
+  *out = a;
+  return KD_OK;
+}
+
+static int adj_save(kd_adj *a, int64_t slot) {
+  kd_ctx *c = a->c;
+  const size_t N = (size_t)a->n;
+  StateDev &s = c->st[c->cur];
+  const double *src[7] = {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m};
+  double *base = a->ck + (size_t)slot * 7 * N;
+  for (int i = 0; i < 7; ++i)
+    HIP_TRY(hipMemcpyAsync(base + i * N, src[i], 8 * N, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(a->ck_oid + (size_t)slot * N, s.oid, 4 * N, hipMemcpyDeviceToDevice,
+                         c->stream));
+  return KD_OK;
+}
+
+static int adj_restore(kd_adj *a, int64_t slot) {
+  kd_ctx *c = a->c;
+  const size_t N = (size_t)a->n;
+  StateDev &s = c->st[c->cur];
+  double *dst[7] = {s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m};
+  const double *base = a->ck + (size_t)slot * 7 * N;
+  for (int i = 0; i < 7; ++i)
+    HIP_TRY(hipMemcpyAsync(dst[i], base + i * N, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipMemcpyAsync(s.oid, a->ck_oid + (size_t)slot * N, 4 * N, hipMemcpyDeviceToDevice,
+                         c->stream));
+  return KD_OK;
+}
+
+static int adj_step_forward(kd_adj *a) {
+  kd_ctx *c = a->c;
+  int rc = build_impl(c);
+  if (rc) return rc;
+  if ((rc = traverse_impl(c))) return rc;
+  HIP_TRY(launch_kick_drift(c->st[c->cur], c->acc[0], c->acc[1], c->acc[2], (uint32_t)c->n,
+                            c->params.dt, c->bad, c->stream));
+  return KD_OK;
+}
+
+extern "C" int kd_adj_forward(kd_adj *a, const double *x0, const double *y0, const double *z0,
+                              const double *vx0, const double *vy0, const double *vz0,
+                              const double *m, double *x, double *y, double *z, double *vx,
+                              double *vy, double *vz) {
+  if (!a) return fail(KD_ERR_ARG, "adjoint context is NULL");
+  kd_ctx *c = a->c;
+  int rc = kd_upload_device(c, a->n, x0, y0, z0, vx0, vy0, vz0, m);
+  if (rc) return rc;
+  c->params.count_interactions = 0;
+  HIP_TRY(hipMemsetAsync(c->bad, 0, sizeof(int), c->stream));
+  for (int64_t k = 0; k < a->steps; ++k) {
+    if (k % a->every == 0 && (rc = adj_save(a, k / a->every))) return rc;
+    if ((rc = adj_step_forward(a))) return rc;
+  }
+  int bad = 0;
+  HIP_TRY(hipMemcpyAsync(&bad, c->bad, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (bad) return fail(KD_ERR_NONFINITE, "a body reached a non-finite position");
+  a->forward_done = true;
+  return kd_download_device(c, x, y, z, vx, vy, vz);
+}
+
+extern "C" int kd_adj_backward(kd_adj *a, const double *gpx, const double *gpy, const double *gpz,
+                               const double *gvx, const double *gvy, const double *gvz,
+                               double *gx0, double *gy0, double *gz0, double *gvx0,
+                               double *gvy0, double *gvz0, double *gm) {
+  if (!a) return fail(KD_ERR_ARG, "adjoint context is NULL");
+  if (!a->forward_done) return fail(KD_ERR_STATE, "kd_adj_forward has not run");
+  kd_ctx *c = a->c;
+  HIP_TRY(hipSetDevice(c->device));
+  const size_t N = (size_t)a->n, S = (size_t)a->slots;
+  const double *gin[6] = {gpx, gpy, gpz, gvx, gvy, gvz};
+  double *acc[6] = {a->lp[0], a->lp[1], a->lp[2], a->lv[0], a->lv[1], a->lv[2]};
+  for (int i = 0; i < 6; ++i) {
+    if (gin[i])
+      HIP_TRY(hipMemcpyAsync(acc[i], gin[i], 8 * N, hipMemcpyDeviceToDevice, c->stream));
+    else
+      HIP_TRY(hipMemsetAsync(acc[i], 0, 8 * N, c->stream));
+  }
+  HIP_TRY(hipMemsetAsync(a->lm, 0, 8 * N, c->stream));
+  const bool mixed = c->params.precision == KD_PREC_MIXED;
+  int rc;
+  for (int64_t seg = a->nck - 1; seg >= 0; --seg) {
+    const int64_t k0 = seg * a->every;
+    const int64_t len = std::min(a->every, a->steps - k0);
+    if ((rc = adj_restore(a, seg))) return rc;
+    for (int64_t j = 0; j < len; ++j) {
+      if ((rc = build_impl(c))) return rc;
+      HIP_TRY(hipMemcpyAsync(a->t_pos64 + j * N, c->pos64, 32 * N, hipMemcpyDeviceToDevice,
+                             c->stream));
+      HIP_TRY(hipMemcpyAsync(a->t_pos32 + j * N, c->pos32, 16 * N, hipMemcpyDeviceToDevice,
+                             c->stream));
+      HIP_TRY(hipMemcpyAsync(a->t_oid + j * N, c->st[c->cur].oid, 4 * N,
+                             hipMemcpyDeviceToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(a->t_rec64 + j * S, c->tree.rec64, sizeof(NodeRec64) * S,
+                             hipMemcpyDeviceToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(a->t_rec32 + j * S, c->tree.rec32, sizeof(NodeRec32) * S,
+                             hipMemcpyDeviceToDevice, c->stream));
+      if (j + 1 < len) {
+        if ((rc = traverse_impl(c))) return rc;
+        HIP_TRY(launch_kick_drift(c->st[c->cur], c->acc[0], c->acc[1], c->acc[2],
+                                  (uint32_t)c->n, c->params.dt, c->bad, c->stream));
+      }
+    }
+    for (int64_t j = len - 1; j >= 0; --j) {
+      AdjStep st;
+      st.rec64 = a->t_rec64 + j * S;
+      st.rec32 = a->t_rec32 + j * S;
+      st.pos64 = a->t_pos64 + j * N;
+      st.pos32 = a->t_pos32 + j * N;
+      st.oid = a->t_oid + j * N;
+      st.seg_count = c->tree.seg_count;
+      for (int d = 0; d < 3; ++d) {
+        st.lp[d] = a->lp[d];
+        st.lv[d] = a->lv[d];
+        st.u[d] = a->u[d];
+      }
+      st.lm = a->lm;
+      st.gnode = a->gnode;
+      st.n = (uint32_t)N;
+      st.slots = (uint32_t)S;
+      st.leaf = (uint32_t)c->params.leaf_size;
+      st.theta = c->params.theta;
+      st.dt = c->params.dt;
+      st.mixed = mixed;
+      HIP_TRY(launch_adj_step(st, c->stream));
+    }
+  }
+  double *outs[7] = {gx0, gy0, gz0, gvx0, gvy0, gvz0, gm};
+  const double *srcs[7] = {a->lp[0], a->lp[1], a->lp[2], a->lv[0], a->lv[1], a->lv[2], a->lm};
+  for (int i = 0; i < 7; ++i)
+    if (outs[i])
+      HIP_TRY(hipMemcpyAsync(outs[i], srcs[i], 8 * N, hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return KD_OK;
+}

@@ -168,6 +168,24 @@ hipError_t launch_traverse_forest(int precision, bool count, const WalkShape &ws
                                   uint32_t *queue, int cus, double theta, int leaf, double *ax,
                                   double *ay, double *az, unsigned long long *counters,
                                   hipStream_t s);
+// One step of the reverse sweep: the step's recorded tree and tree-order
+// bodies, and the adjoint arrays (original body order).
+struct AdjStep {
+  const NodeRec64 *rec64;
+  const NodeRec32 *rec32;
+  const double4 *pos64;
+  const float4 *pos32;
+  const uint32_t *oid;
+  const uint32_t *seg_count;
+  double *lp[3], *lv[3], *u[3];
+  double *lm;
+  double *gnode;  // 4 * slots: d/dc (x, y, z) and d/dM per node
+  uint32_t n, slots, leaf;
+  double theta, dt;
+  bool mixed;
+};
+hipError_t launch_adj_step(const AdjStep &st, hipStream_t s);
+
 // Replace every memset node of a captured graph by a fill kernel node with
 // the same dependencies (memset nodes are not replayed reliably here).
 hipError_t graph_replace_memsets(hipGraph_t g, int *replaced);

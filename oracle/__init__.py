@@ -92,6 +92,14 @@ def lib():
         L.orc_compact_count.argtypes = [C.c_int64, C.c_int]
         L.orc_compact_count.restype = C.c_int64
         L.orc_kick_drift.argtypes = [C.c_int64, _d, _d, _d, _d, _d, _d, _d, _d, _d, C.c_double]
+        L.orc_adj_forward.argtypes = [C.c_int64] + [_d] * 7 + [C.c_double, C.c_int64, C.c_double,
+                                                              C.c_int, C.POINTER(C.c_void_p),
+                                                              C.c_void_p]
+        L.orc_adj_forward.restype = C.c_int
+        L.orc_adj_backward.argtypes = [C.c_void_p] + [_d] * 14
+        L.orc_adj_backward.restype = C.c_int
+        L.orc_tape_free.argtypes = [C.c_void_p]
+        L.orc_adj_debug_drop.argtypes = [C.c_int]
         _ORC = L
     return _ORC
 
@@ -201,6 +209,50 @@ def simple_sim(b: dict, dt=1e-3, steps=1, theta=0.3, leaf=7, layout=LAYOUT_COMPA
     if rc:
         raise RuntimeError(f"orc_simple_sim failed: {rc}")
     return (arrs, hist) if keep_acc else arrs
+
+
+class Tape:
+    """A recorded frozen-topology run (orc_tape)."""
+
+    def __init__(self, ptr, n, m):
+        self.ptr, self.n, self.m = ptr, n, m
+
+    def __del__(self):
+        if self.ptr:
+            lib().orc_tape_free(self.ptr)
+            self.ptr = None
+
+
+def adj_forward(b: dict, dt=1e-3, steps=1, theta=0.3, leaf=7, replay: Tape | None = None):
+    """Frozen-topology simple_sim.  Without replay: records and returns
+    (final state, tape); with replay: (final state, None) using that tape's
+    trees and interaction lists."""
+    L = lib()
+    arrs = {k: _f(b, k).copy() for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    n = len(arrs["x"])
+    tp = C.c_void_p()
+    rc = L.orc_adj_forward(n, *[_d_(arrs[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "m")],
+                           dt, steps, theta, leaf, None if replay else C.byref(tp),
+                           replay.ptr if replay else None)
+    if rc:
+        raise RuntimeError(f"orc_adj_forward failed: {rc}")
+    return arrs, (None if replay else Tape(tp, n, arrs["m"]))
+
+
+def adj_backward(tape: Tape, lp, lv):
+    """Gradients (d/dp0 (n,3), d/dv0 (n,3), d/dm (n,)) from dL/dp_N and
+    dL/dv_N (both (n,3))."""
+    L = lib()
+    n = tape.n
+    lp = np.ascontiguousarray(np.asarray(lp, np.float64).T)
+    lv = np.ascontiguousarray(np.asarray(lv, np.float64).T)
+    out = [np.zeros(n) for _ in range(7)]
+    rc = L.orc_adj_backward(tape.ptr, _d_(np.ascontiguousarray(tape.m)),
+                            *[_d_(lp[c]) for c in range(3)], *[_d_(lv[c]) for c in range(3)],
+                            *[_d_(o) for o in out])
+    if rc:
+        raise RuntimeError(f"orc_adj_backward failed: {rc}")
+    return np.stack(out[0:3], 1), np.stack(out[3:6], 1), out[6]
 
 
 def direct(b: dict, idx):

@@ -130,6 +130,30 @@ void orc_kick_drift(int64_t n, double *x, double *y, double *z, double *vx,
                     double *vy, double *vz, const double *ax, const double *ay,
                     const double *az, double dt);
 
+/* ---- the differentiated step (kdtree_adjoint.c) ----
+ * simple_sim with the tree topology frozen per step: step k's tree and
+ * every body's interaction list come from the positions at the start of the
+ * step, node moments are recomputed from the members.  A tape records the
+ * trees, lists and start positions of every step. */
+typedef struct orc_tape orc_tape;
+/* steps of the frozen-topology step, updating (x..vz) in place.  record:
+ * receives a new tape (build + lists from the run's own positions).
+ * replay: use that tape's trees and lists instead (the same function at
+ * other inputs).  One of the two must be given. */
+int orc_adj_forward(int64_t n, double *x, double *y, double *z, double *vx, double *vy,
+                    double *vz, const double *m, double dt, int64_t steps, double theta,
+                    int leaf_size, orc_tape **record, const orc_tape *replay);
+/* Reverse sweep: given dL/dp_N and dL/dv_N, the gradients with respect to
+ * the initial positions, initial velocities and masses. */
+int orc_adj_backward(const orc_tape *t, const double *m, const double *lpx, const double *lpy,
+                     const double *lpz, const double *lvx, const double *lvy, const double *lvz,
+                     double *gx0, double *gy0, double *gz0, double *gvx0, double *gvy0,
+                     double *gvz0, double *gm);
+void orc_tape_free(orc_tape *t);
+/* Test hook: plant a mistake in the reverse sweep (1: drop node-to-member
+ * terms, 2: drop P2P partner terms, 3: drop mass terms; 0: none). */
+void orc_adj_debug_drop(int which);
+
 #ifdef __cplusplus
 }
 #endif
