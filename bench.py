@@ -157,6 +157,93 @@ def run_batch(nsys, steps, loop_steps, theta, leaf, dt, seed):
                 unit="particle-steps/s", precision="mixed", graph=True)
 
 
+def run_adjoint(n, steps, every, theta, leaf, dt, seed, ndir, ndir64, cpu_n):
+    """Differentiation workload: gradients of sum |p_N - p_ref|^2 (p_ref from
+    velocities + N(0, 1e-4)) with respect to initial velocities and masses,
+    mixed precision, checkpoint every `every` steps; checked against central
+    differences (eps 1e-7) of GPU forward runs along seeded random
+    directions, and (at cpu_n bodies) against the oracle's FP64 reference
+    gradient."""
+    import torch
+    from multilanguagekdtree_amd.autograd import Adjoint
+    import oracle
+
+    def cuda(a):
+        return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64, device="cuda")
+
+    P = lambda s: np.stack([s["x"], s["y"], s["z"]], 1)  # noqa: E731
+    V = lambda s: np.stack([s["vx"], s["vy"], s["vz"]], 1)  # noqa: E731
+
+    def final_positions(b, prec):
+        with K.Simulation(len(b["x"]), theta=theta, dt=dt, leaf_size=leaf, precision=prec) as s:
+            s.upload(b)
+            s.step(steps)
+            return P(s.download())
+
+    def grads(b, pref, prec, timing=False):
+        sim = Adjoint(len(b["x"]), steps, every, theta=theta, dt=dt, leaf_size=leaf,
+                      precision=prec)
+        x0, v0, m = cuda(P(b)), cuda(V(b)), cuda(b["m"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pN, _ = sim.forward(x0, v0, m)
+        t1 = time.perf_counter()
+        gx, gv, gm = sim.backward(2 * (pN - cuda(pref)), None)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        sim.close()
+        return gv.cpu().numpy(), gm.cpu().numpy(), t1 - t0, t2 - t1
+
+    rng = np.random.default_rng(seed + 7)
+    b = K.ring(n, seed=seed + 3)
+    ref = dict(b)
+    for k in ("vx", "vy", "vz"):
+        ref[k] = b[k] + rng.normal(size=n) * 1e-4
+    out = {"n": n, "steps": steps, "checkpoint_every": every, "precision": "mixed"}
+    for name, prec, nd in (("mixed", K.KD_PREC_MIXED, ndir), ("fp64", K.KD_PREC_FP64, ndir64)):
+        pref = final_positions(ref, prec)
+        grads(b, pref, prec)  # warm-up
+        gv, gm, tf, tb = grads(b, pref, prec)
+        devs = []
+        for _ in range(nd):
+            dv = rng.normal(size=(n, 3))
+            dm = b["m"] * rng.normal(size=n)
+            eps = 1e-7
+
+            def loss(sgn):
+                c = dict(b)
+                c["vx"], c["vy"], c["vz"] = (V(b) + sgn * eps * dv).T
+                c["m"] = b["m"] + sgn * eps * dm
+                return ((final_positions(c, prec) - pref) ** 2).sum()
+            fd = (loss(1) - loss(-1)) / (2 * eps)
+            ad = float((gv * dv).sum() + (gm * dm).sum())
+            devs.append(abs(fd - ad) / abs(ad))
+        out[name] = {"forward_s": tf, "backward_s": tb,
+                     "forward_particle_steps_per_s": n * steps / tf,
+                     "fwd_bwd_particle_steps_per_s": n * steps / (tf + tb),
+                     "fd_directions": nd,
+                     "fd_rel_dev_median": float(np.median(devs)) if devs else None,
+                     "fd_rel_dev_max": float(np.max(devs)) if devs else None}
+        log(f"[bench] adjoint {name}: {out[name]}")
+    # FP64 CPU reference gradient (the oracle, frozen topology) at cpu_n bodies
+    bc = K.ring(cpu_n, seed=seed + 5)
+    refc = dict(bc)
+    for k in ("vx", "vy", "vz"):
+        refc[k] = bc[k] + rng.normal(size=cpu_n) * 1e-4
+    prc, _ = oracle.adj_forward(refc, dt, steps, theta, leaf)
+    fin, tape = oracle.adj_forward(bc, dt, steps, theta, leaf)
+    _, ov, om = oracle.adj_backward(tape, 2 * (P(fin) - P(prc)), np.zeros((cpu_n, 3)))
+    del tape
+    cmp = {}
+    for name, prec in (("mixed", K.KD_PREC_MIXED), ("fp64", K.KD_PREC_FP64)):
+        gv, gm, _, _ = grads(bc, P(prc), prec)
+        cmp[name] = {"v_rel": float(np.linalg.norm(gv - ov) / np.linalg.norm(ov)),
+                     "m_rel": float(np.linalg.norm(gm - om) / np.linalg.norm(om))}
+    out["vs_cpu_fp64_gradient"] = dict(n=cpu_n, steps=steps, **cmp)
+    log(f"[bench] adjoint vs CPU FP64 gradient: {out['vs_cpu_fp64_gradient']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,7 +257,14 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp64", "both", "batch"], default="both")
+    ap.add_argument("--only", choices=["mixed", "fp64", "both", "batch", "adjoint"],
+                    default="both")
+    ap.add_argument("--adj-n", type=int, default=100_000)
+    ap.add_argument("--adj-steps", type=int, default=200)
+    ap.add_argument("--adj-every", type=int, default=20)
+    ap.add_argument("--adj-dirs", type=int, default=64)
+    ap.add_argument("--adj-dirs-fp64", type=int, default=8)
+    ap.add_argument("--adj-cpu-n", type=int, default=2000)
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-loop-steps", type=int, default=2)
@@ -182,6 +276,12 @@ def main():
         sys.exit(2)
 
     n, dt = args.n, 1e-3
+    if args.only == "adjoint":
+        log(f"[bench] adjoint: n={args.adj_n} steps={args.adj_steps}")
+        ra = run_adjoint(args.adj_n, args.adj_steps, args.adj_every, args.theta, args.leaf, dt,
+                         args.seed, args.adj_dirs, args.adj_dirs_fp64, args.adj_cpu_n)
+        print(json.dumps(ra), flush=True)
+        return
     if args.only == "batch":
         log(f"[bench] batch of {args.batch_systems} systems, {args.batch_steps} steps")
         rb = run_batch(args.batch_systems, args.batch_steps, args.batch_loop_steps, args.theta,
