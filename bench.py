@@ -257,8 +257,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp64", "both", "batch", "adjoint"],
+    ap.add_argument("--only", choices=["mixed", "fp64", "both", "batch", "adjoint", "copy"],
                     default="both")
+    ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--adj-n", type=int, default=100_000)
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-every", type=int, default=20)
@@ -276,6 +277,19 @@ def main():
         sys.exit(2)
 
     n, dt = args.n, 1e-3
+    if args.only == "copy":
+        # a plain device-to-device copy of known size: the calibration run for
+        # the FETCH_SIZE / WRITE_SIZE counters (one kernel per copy)
+        import torch
+        nbytes = args.copy_mib << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                          "copies": 3}), flush=True)
+        return
     if args.only == "adjoint":
         log(f"[bench] adjoint: n={args.adj_n} steps={args.adj_steps}")
         ra = run_adjoint(args.adj_n, args.adj_steps, args.adj_every, args.theta, args.leaf, dt,
@@ -313,11 +327,15 @@ def main():
     flops = 10 * r["tests"] + 9 * r["m2p"] + 18 * r["p2p"]
     achieved = flops / (r["traverse_ms"] * 1e-3) / 1e12
     peak = PEAK_FP32_VALU_TFLOPS if main_name == "mixed" else PEAK_FP64_VALU_TFLOPS
+    # HBM bytes per launch of the walk: FETCH_SIZE and WRITE_SIZE counter
+    # passes, each scaled by what the counter reads for a plain copy of known
+    # size (profiles/traffic_ring10M_mixed.md)
     traffic = None
     tf = ROOT / "profiles" / "traffic.json"
     if tf.exists():
         try:
-            traffic = json.loads(tf.read_text()).get(f"traverse_{main_name}_n{n}")
+            t = json.loads(tf.read_text()).get(f"traverse_{main_name}_n{n}")
+            traffic = None if t is None else float(t["bytes"])
         except Exception:
             traffic = None
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
