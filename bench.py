@@ -244,6 +244,43 @@ def run_adjoint(n, steps, every, theta, leaf, dt, seed, ndir, ndir64, cpu_n):
     return out
 
 
+def run_build_only(n, leaf, seed, reps):
+    """Build-only benchmark after Parallel/RustVersion/src/bin/bench_build_tree.rs:6-41:
+    circular_orbits(n) (n + 1 bodies), tree build alone; GPU build from the
+    initial-condition order (first) and from the previous tree order (steady
+    state), beside the reference C sequential build_tree of the same bodies."""
+    import ctypes as C
+    import oracle
+    b = K.circular_orbits(n, seed=seed)
+    nb = len(b["x"])
+    with K.Simulation(nb, leaf_size=leaf) as s:
+        s.upload(b)
+        t0 = time.perf_counter()
+        s.build()
+        t1 = time.perf_counter()
+        times = []
+        for _ in range(reps):
+            t2 = time.perf_counter()
+            s.build()
+            times.append(time.perf_counter() - t2)
+    out = {"bodies": nb, "leaf_size": leaf, "gpu_first_build_ms": 1e3 * (t1 - t0),
+           "gpu_build_ms_median": 1e3 * float(np.median(times)), "reps": reps}
+    R = oracle.ref_lib("par")
+    if R is not None:
+        f = {k: np.ascontiguousarray(b[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+        d = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
+        idx = np.zeros(1, np.int64)
+        acc = np.empty(3)
+        t = np.zeros(3)
+        R.ref_cpu_sample(nb, d(f["x"]), d(f["y"]), d(f["z"]), d(f["vx"]), d(f["vy"]), d(f["vz"]),
+                         d(f["m"]), 1, idx.ctypes.data_as(C.POINTER(C.c_int64)), 1, 1e-3,
+                         d(acc), d(t))
+        out["cpu_reference_build_s"] = float(t[0])
+        out["cpu_reference"] = "Parallel/CVersion build_tree (sequential, leaf 7, gcc -O2), 1 thread"
+        out["speedup_vs_cpu"] = float(t[0]) / (out["gpu_build_ms_median"] * 1e-3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,9 +294,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
-    ap.add_argument("--only", choices=["mixed", "fp64", "both", "batch", "adjoint", "copy"],
-                    default="both")
+    ap.add_argument("--only", choices=["mixed", "fp64", "both", "batch", "adjoint", "copy",
+                                       "build"], default="both")
     ap.add_argument("--copy-mib", type=int, default=1024)
+    # MAX_PARTS of the crate the build-only benchmark comes from
+    # (Parallel/RustVersion/src/array_kd_tree.rs:14)
+    ap.add_argument("--build-leaf", type=int, default=8)
     ap.add_argument("--adj-n", type=int, default=100_000)
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-every", type=int, default=20)
@@ -277,6 +317,10 @@ def main():
         sys.exit(2)
 
     n, dt = args.n, 1e-3
+    if args.only == "build":
+        rb = run_build_only(args.n, args.build_leaf, args.seed, 5)
+        print(json.dumps(rb), flush=True)
+        return
     if args.only == "copy":
         # a plain device-to-device copy of known size: the calibration run for
         # the FETCH_SIZE / WRITE_SIZE counters (one kernel per copy)

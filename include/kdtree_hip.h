@@ -132,6 +132,15 @@ int kd_accel_tree(int64_t n, const double *x, const double *y, const double *z,
                   const double *m, const kd_tree_view *tree, double theta, int64_t k,
                   const int64_t *idx, double *ax, double *ay, double *az);
 
+/* print_tree (Parallel/RustVersion/src/array_kd_tree.rs:666-692): write a node
+ * vector to `path` in the reference's text format (read by
+ * TreeVisualizer/src/main/scala/ViewTrees.scala:32-80): the node count, then
+ * per node "I split_dim split_val left right" or "L k" and k lines "x y z" of
+ * the members' positions (x, y, z indexed by body).  Numbers as Rust's `{}`
+ * prints them (shortest round-trip decimal, no exponent).  Host only. */
+int kd_print_tree(const kd_tree_view *tree, int64_t n, const double *x, const double *y,
+                  const double *z, const char *path);
+
 /* Build, then accelerations of every body: calc_accel(p, ..) for all p
  * (array_kd_tree.rs:619 / Sequential :174), in original order. */
 int kd_accel(kd_ctx *ctx, double *ax, double *ay, double *az);
@@ -170,6 +179,23 @@ int kd_reset_timings(kd_ctx *ctx);
 int kd_direct_sum(kd_ctx *ctx, int64_t k, const int64_t *idx, double *ax,
                   double *ay, double *az, double *absx, double *absy,
                   double *absz);
+
+/* ---- standalone selection ----
+ * quickstat_index(indices, goal, |i, j| vals[i] < vals[j])
+ * (Parallel/RustVersion/src/quickstat.rs:9-34): reorder indices[0..len) so
+ * that the element of rank goal under the order (vals[i], i) sits at goal,
+ * every element before it compares less and every element after it greater
+ * (the reference's postcondition, quickstat.rs:199-215).  Both sides keep
+ * their input order.  Host arrays; indices must lie in [0, nvals). */
+int kd_quickstat_index(int64_t len, int64_t *indices, int64_t goal, const double *vals,
+                       int64_t nvals);
+/* The same on device arrays, with a caller-owned device workspace of at
+ * least kd_quickstat_workspace_bytes(len) bytes.  No counterpart in the
+ * reference beyond quickstat_index. */
+int64_t kd_quickstat_workspace_bytes(int64_t len);
+int kd_quickstat_index_device(int64_t len, int64_t *indices, int64_t goal,
+                              const double *vals, int64_t nvals, void *workspace,
+                              int64_t workspace_bytes);
 
 /* ---- batches: many independent systems in one context ----
  * A batch holds nsys independent systems of n[i] bodies each and runs

@@ -84,6 +84,7 @@ SIGNATURES = {
     "kd_nodes_needed": (C.c_int64, [C.c_int64, C.c_int32, C.c_int32]),
     "kd_accel_tree": (C.c_int, [C.c_int64] + [_d] * 4 + [_P(KdTreeView), C.c_double,
                                                           C.c_int64, _i64, _d, _d, _d]),
+    "kd_print_tree": (C.c_int, [_P(KdTreeView), C.c_int64, _d, _d, _d, C.c_char_p]),
     "kd_accel": (C.c_int, [C.c_void_p, _d, _d, _d]),
     "kd_accel_device": (C.c_int, [C.c_void_p] + [C.c_void_p] * 3),
     "kd_step": (C.c_int, [C.c_void_p, C.c_int64]),
@@ -93,6 +94,10 @@ SIGNATURES = {
     "kd_timings": (C.c_int, [C.c_void_p, _d]),
     "kd_reset_timings": (C.c_int, [C.c_void_p]),
     "kd_direct_sum": (C.c_int, [C.c_void_p, C.c_int64, _i64] + [_d] * 6),
+    "kd_quickstat_index": (C.c_int, [C.c_int64, _i64, C.c_int64, _d, C.c_int64]),
+    "kd_quickstat_workspace_bytes": (C.c_int64, [C.c_int64]),
+    "kd_quickstat_index_device": (C.c_int, [C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
+                                            C.c_int64, C.c_void_p, C.c_int64]),
     "kd_batch_create": (C.c_int, [C.c_int64, _i64, _P(KdParams), _P(C.c_void_p)]),
     "kd_batch_destroy": (C.c_int, [C.c_void_p]),
     "kd_batch_size": (C.c_int64, [C.c_void_p]),

@@ -1562,3 +1562,142 @@ extern "C" int kd_adj_backward(kd_adj *a, const double *gpx, const double *gpy, 
   HIP_TRY(hipStreamSynchronize(c->stream));
   return KD_OK;
 }
+
+// ---------------------------------------------------------------------------
+// tree dump (text interchange format)
+// ---------------------------------------------------------------------------
+
+// The shortest decimal that reads back as v, written positionally (no
+// exponent), as Rust's `{}` prints an f64: 0.1 -> "0.1", 1.0 -> "1",
+// 1e-7 -> "0.0000001", -0.0 -> "-0".
+static std::string fmt_f64(double v) {
+  if (std::isnan(v)) return "NaN";
+  if (std::isinf(v)) return v < 0 ? "-inf" : "inf";
+  if (v == 0.0) return std::signbit(v) ? "-0" : "0";
+  char buf[64];
+  int p = 1;
+  for (; p <= 17; ++p) {
+    snprintf(buf, sizeof buf, "%.*e", p - 1, v);
+    if (strtod(buf, nullptr) == v) break;
+  }
+  // buf = [-]d[.ddd]e[+-]xx
+  std::string s = buf;
+  const bool neg = s[0] == '-';
+  if (neg) s.erase(0, 1);
+  const size_t e = s.find('e');
+  const int exp10 = atoi(s.c_str() + e + 1);
+  std::string digits;
+  for (size_t i = 0; i < e; ++i)
+    if (s[i] != '.') digits += s[i];
+  while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+  std::string out;
+  if (exp10 >= 0) {
+    const size_t intlen = (size_t)exp10 + 1;
+    if (digits.size() <= intlen) {
+      out = digits + std::string(intlen - digits.size(), '0');
+    } else {
+      out = digits.substr(0, intlen) + "." + digits.substr(intlen);
+    }
+  } else {
+    out = "0." + std::string((size_t)(-exp10 - 1), '0') + digits;
+  }
+  return neg ? "-" + out : out;
+}
+
+// print_tree (Parallel/RustVersion/src/array_kd_tree.rs:666-692): the node
+// count, then per node in vector order "I split_dim split_val left right" or
+// "L k" followed by the k member positions "x y z".
+extern "C" int kd_print_tree(const kd_tree_view *t, int64_t n, const double *x, const double *y,
+                             const double *z, const char *path) {
+  if (!t || !x || !y || !z || !path || n < 0) return fail(KD_ERR_ARG, "bad argument to kd_print_tree");
+  if (!t->is_leaf || !t->num_parts || !t->first || !t->perm || !t->split_dim || !t->split_val ||
+      !t->left || !t->right)
+    return fail(KD_ERR_ARG, "incomplete tree view");
+  FILE *f = fopen(path, "w");
+  if (!f) return fail(KD_ERR_ARG, "cannot open %s for writing", path);
+  std::string line;
+  bool ok = fprintf(f, "%lld\n", (long long)t->n_nodes) > 0;
+  for (int64_t i = 0; ok && i < t->n_nodes; ++i) {
+    if (t->is_leaf[i]) {
+      const int64_t k = t->num_parts[i], f0 = t->first[i];
+      if (k < 0 || f0 < 0 || f0 + k > n) {
+        fclose(f);
+        return fail(KD_ERR_ARG, "leaf %lld members out of range", (long long)i);
+      }
+      ok = fprintf(f, "L %lld\n", (long long)k) > 0;
+      for (int64_t q = 0; ok && q < k; ++q) {
+        const int64_t p = t->perm[f0 + q];
+        if (p < 0 || p >= n) {
+          fclose(f);
+          return fail(KD_ERR_ARG, "leaf member index out of range");
+        }
+        line = fmt_f64(x[p]) + " " + fmt_f64(y[p]) + " " + fmt_f64(z[p]) + "\n";
+        ok = fputs(line.c_str(), f) >= 0;
+      }
+    } else {
+      line = "I " + std::to_string(t->split_dim[i]) + " " + fmt_f64(t->split_val[i]) + " " +
+             std::to_string(t->left[i]) + " " + std::to_string(t->right[i]) + "\n";
+      ok = fputs(line.c_str(), f) >= 0;
+    }
+  }
+  if (fclose(f) != 0) ok = false;
+  if (!ok) return fail(KD_ERR_ARG, "write to %s failed", path);
+  return KD_OK;
+}
+
+// ---------------------------------------------------------------------------
+// standalone selection
+// ---------------------------------------------------------------------------
+
+extern "C" int64_t kd_quickstat_workspace_bytes(int64_t len) {
+  return len < 1 ? -1 : (int64_t)select_workspace_bytes(len);
+}
+
+extern "C" int kd_quickstat_index_device(int64_t len, int64_t *indices, int64_t goal,
+                                         const double *vals, int64_t nvals, void *workspace,
+                                         int64_t workspace_bytes) {
+  if (len < 1 || !indices || !vals || nvals < 1 || !workspace)
+    return fail(KD_ERR_ARG, "bad argument to kd_quickstat_index_device");
+  if (goal < 0 || goal >= len) return fail(KD_ERR_ARG, "goal must be in [0, len)");
+  if (workspace_bytes < (int64_t)select_workspace_bytes(len))
+    return fail(KD_ERR_CAPACITY, "workspace smaller than kd_quickstat_workspace_bytes(len)");
+  hipError_t e = launch_select(indices, vals, len, nvals, goal, workspace, nullptr);
+  if (e == hipErrorInvalidValue) return fail(KD_ERR_ARG, "an index is outside [0, nvals)");
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) return fail(KD_ERR_HIP, "selection: %s", hipGetErrorString(e));
+  return KD_OK;
+}
+
+extern "C" int kd_quickstat_index(int64_t len, int64_t *indices, int64_t goal,
+                                  const double *vals, int64_t nvals) {
+  if (len < 1 || !indices || !vals || nvals < 1)
+    return fail(KD_ERR_ARG, "bad argument to kd_quickstat_index");
+  if (goal < 0 || goal >= len) return fail(KD_ERR_ARG, "goal must be in [0, len)");
+  for (int64_t i = 0; i < len; ++i)
+    if (indices[i] < 0 || indices[i] >= nvals)
+      return fail(KD_ERR_ARG, "indices[%lld] is outside [0, nvals)", (long long)i);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(KD_ERR_NODEVICE, "no HIP device visible");
+  const size_t wsb = select_workspace_bytes(len);
+  void *di = nullptr, *dv = nullptr, *ws = nullptr;
+  hipError_t e = hipMalloc(&di, 8 * (size_t)len);
+  if (e == hipSuccess) e = hipMalloc(&dv, 8 * (size_t)nvals);
+  if (e == hipSuccess) e = hipMalloc(&ws, wsb);
+  if (e == hipSuccess) e = hipMemcpy(di, indices, 8 * (size_t)len, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dv, vals, 8 * (size_t)nvals, hipMemcpyHostToDevice);
+  int rc = KD_OK;
+  if (e == hipSuccess) {
+    rc = kd_quickstat_index_device(len, (int64_t *)di, goal, (const double *)dv, nvals, ws,
+                                   (int64_t)wsb);
+    if (rc == KD_OK) e = hipMemcpy(indices, di, 8 * (size_t)len, hipMemcpyDeviceToHost);
+  }
+  if (di) (void)hipFree(di);
+  if (dv) (void)hipFree(dv);
+  if (ws) (void)hipFree(ws);
+  if (rc) return rc;
+  if (e != hipSuccess)
+    return fail(e == hipErrorOutOfMemory ? KD_ERR_NOMEM : KD_ERR_HIP, "selection: %s",
+                hipGetErrorString(e));
+  return KD_OK;
+}

@@ -186,6 +186,11 @@ struct AdjStep {
 };
 hipError_t launch_adj_step(const AdjStep &st, hipStream_t s);
 
+// Standalone selection (quickstat_index): workspace size and launcher.
+size_t select_workspace_bytes(int64_t n);
+hipError_t launch_select(int64_t *idx, const double *vals, int64_t n, int64_t nvals,
+                         int64_t goal, void *ws, hipStream_t s);
+
 // Replace every memset node of a captured graph by a fill kernel node with
 // the same dependencies (memset nodes are not replayed reliably here).
 hipError_t graph_replace_memsets(hipGraph_t g, int *replaced);

@@ -349,6 +349,29 @@ def calc_accel_tree(bodies: dict, tree: Tree, theta: float = 0.3,
     return np.stack(out, axis=1)
 
 
+def quickstat_index(indices, goal: int, vals) -> np.ndarray:
+    """quickstat_index(indices, goal, |i, j| vals[i] < vals[j])
+    (Parallel/RustVersion/src/quickstat.rs:9-34) on the GPU; returns the
+    reordered indices (ties in vals ordered by index)."""
+    ix = np.ascontiguousarray(indices, np.int64).copy()
+    v = np.ascontiguousarray(vals, np.float64)
+    check(_lib.lib().kd_quickstat_index(len(ix), iptr(ix), goal, dptr(v), len(v)))
+    return ix
+
+
+def print_tree(step: int, tree: Tree, bodies: dict, directory: str = ".") -> str:
+    """print_tree (Parallel/RustVersion/src/array_kd_tree.rs:666-692): writes
+    tree{step}.txt in `directory` and returns its path."""
+    import os
+    path = os.path.join(directory, f"tree{step}.txt")
+    cols = [np.ascontiguousarray(bodies[f], np.float64) for f in ("x", "y", "z")]
+    v, keep = tree.view()
+    check(_lib.lib().kd_print_tree(C.byref(v), len(cols[0]), *[dptr(c) for c in cols],
+                                   path.encode()))
+    del keep
+    return path
+
+
 def circular_orbits(n: int, seed: int = 0) -> dict:
     """circular_orbits(n) of array_particle.rs:19-44 (n+1 bodies), seeded."""
     return _lib.ic("circular_orbits", n, seed)
@@ -365,6 +388,6 @@ def ring(n: int, seed: int = 0) -> dict:
 
 
 __all__ = ["Simulation", "Batch", "ensemble", "Tree", "allocate_node_vec", "simple_sim", "build_tree",
-           "calc_accel", "calc_accel_tree", "circular_orbits", "two_bodies", "ring",
+           "calc_accel", "calc_accel_tree", "print_tree", "quickstat_index", "circular_orbits", "two_bodies", "ring",
            "KD_PREC_FP64", "KD_PREC_MIXED", "KD_PREC_MIXED_D64",
            "KD_LAYOUT_COMPACT", "KD_LAYOUT_PADDED"]
