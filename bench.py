@@ -281,6 +281,47 @@ def run_build_only(n, leaf, seed, reps):
     return out
 
 
+def run_select(n, seed, reps):
+    """Standalone selection after Parallel/RustVersion/src/bin/bench_quickstat.rs:6-49:
+    n uniform values, indices 0..n, one random goal; GPU selection on
+    device-resident arrays beside the oracle's sequential quickselect."""
+    import ctypes as C
+    import torch
+    import oracle
+    from multilanguagekdtree_amd import _lib
+    rng = np.random.default_rng(seed)
+    vals = rng.random(n)
+    goal = int(rng.integers(0, n))
+    L = _lib.lib()
+    dv = torch.as_tensor(vals, device="cuda")
+    base = torch.arange(n, dtype=torch.int64, device="cuda")
+    di = base.clone()
+    wsb = int(L.kd_quickstat_workspace_bytes(n))
+    ws = torch.empty(wsb, dtype=torch.uint8, device="cuda")
+    times = []
+    for r in range(reps + 1):
+        di.copy_(base)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(L.kd_quickstat_index_device(n, C.c_void_p(di.data_ptr()), goal,
+                                               C.c_void_p(dv.data_ptr()), n,
+                                               C.c_void_p(ws.data_ptr()), wsb))
+        t1 = time.perf_counter()
+        if r:
+            times.append(t1 - t0)
+    pivot = int(di[goal].item())
+    # rank check: exactly `goal` values below the pivot's (values are distinct)
+    assert int((dv < dv[pivot]).sum().item()) == goal
+    del dv, di, ws, base
+    t0 = time.perf_counter()
+    ref = oracle.quickstat_index(np.arange(n), goal, vals)
+    t_cpu = time.perf_counter() - t0
+    assert int(ref[goal]) == pivot
+    return {"n": n, "goal": goal, "gpu_s_median": float(np.median(times)), "reps": reps,
+            "cpu_oracle_s": t_cpu, "cpu": "oracle quickselect (C, 1 thread, seeded pivots)",
+            "speedup_vs_cpu": t_cpu / float(np.median(times))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -295,7 +336,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp64", "both", "batch", "adjoint", "copy",
-                                       "build"], default="both")
+                                       "build", "select"], default="both")
+    ap.add_argument("--select-n", type=int, default=100_000_000)
     ap.add_argument("--copy-mib", type=int, default=1024)
     # MAX_PARTS of the crate the build-only benchmark comes from
     # (Parallel/RustVersion/src/array_kd_tree.rs:14)
@@ -317,6 +359,9 @@ def main():
         sys.exit(2)
 
     n, dt = args.n, 1e-3
+    if args.only == "select":
+        print(json.dumps(run_select(args.select_n, args.seed, 3)), flush=True)
+        return
     if args.only == "build":
         rb = run_build_only(args.n, args.build_leaf, args.seed, 5)
         print(json.dumps(rb), flush=True)
