@@ -342,11 +342,35 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_flags(Lists L, TreeDev t0
   }
 }
 
+// The left/right flags packed to one bit per id: 1/8 of the bytes, so the
+// random reads of the count and scatter passes stay in L2.
+__global__ void k_level_pack(const uint8_t *__restrict__ flags, uint32_t *__restrict__ bits,
+                             uint32_t words) {
+  const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= words) return;
+  const uint4 *src = reinterpret_cast<const uint4 *>(flags) + 2 * (size_t)w;
+  const uint4 a = src[0], b = src[1];
+  const uint32_t q[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  uint32_t v = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    // bytes are 0 or 1: gather bit 0 of each byte
+    const uint32_t x = q[i];
+    const uint32_t nib = (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
+    v |= nib << (4 * i);
+  }
+  bits[w] = v;
+}
+
+__device__ __forceinline__ uint32_t flag_bit(const uint32_t *__restrict__ bits, uint32_t id) {
+  return (bits[id >> 5] >> (id & 31u)) & 1u;
+}
+
 __global__ __launch_bounds__(LVL_THREADS) void k_level_count(Lists L, TreeDev t0,
                                                              const SysDesc *sys,
                                                              uint32_t level,
                                                              uint32_t T,
-                                                             const uint8_t *flags,
+                                                             const uint32_t *__restrict__ bits,
                                                              uint32_t *cnt) {
   __shared__ uint32_t sh[LVL_THREADS / 64];
   const SysDesc S = sys[blockIdx.y];
@@ -366,7 +390,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_count(Lists L, TreeDev t0
       const uint32_t *ld = L.l[d];
       for (int i = 0; i < LVL_ITEMS; ++i) {
         uint32_t j = base + i * LVL_THREADS + threadIdx.x;
-        if (j < e) local += flags[ld[j]];
+        if (j < e) local += flag_bit(bits, ld[j]);
       }
     }
     uint32_t tot;
@@ -398,7 +422,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(Lists L, Lists Lo
                                                                const SysDesc *sys,
                                                                uint32_t level,
                                                                uint32_t T,
-                                                               const uint8_t *flags,
+                                                               const uint32_t *__restrict__ bits,
                                                                const uint32_t *cnt) {
   __shared__ uint32_t sh[LVL_THREADS / 64];
   const SysDesc S = sys[blockIdx.y];
@@ -429,7 +453,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(Lists L, Lists Lo
     for (int i = 0; i < LVL_ITEMS; ++i) {
       uint32_t j = base + i;
       id[i] = j < e ? ld[j] : 0;
-      f[i] = j < e ? flags[id[i]] : 0;
+      f[i] = j < e ? flag_bit(bits, id[i]) : 0;
       local += f[i];
     }
     uint32_t tot;
@@ -993,10 +1017,11 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
     k_level_tiefix<<<dim3(nseg, ny), 1024, 0, s>>>(L, Lb, P, t, f.sys, in.oid, lev, b.tieflag,
                                                    tf_stride);
     k_level_flags<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, t, f.sys, lev, T, b.flags);
-    k_level_count<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, t, f.sys, lev, T, b.flags,
+    k_level_pack<<<(b.flag_words + 255) / 256, 256, 0, s>>>(b.flags, b.bits, b.flag_words);
+    k_level_count<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, t, f.sys, lev, T, b.bits,
                                                              b.cnt);
     k_level_scan<<<dim3(3 * nseg, ny), 256, 0, s>>>(b.cnt, T, f.sys, lev);
-    k_level_scatter<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, Lb, t, f.sys, lev, T, b.flags,
+    k_level_scatter<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, Lb, t, f.sys, lev, T, b.bits,
                                                                b.cnt);
     Lists tmp = L;
     L = Lb;

@@ -135,7 +135,9 @@ struct BuildBufs {
   uint32_t *ids_in;
   void *sort_tmp;
   size_t sort_tmp_bytes;
-  uint8_t *flags;    // left/right flag per working id
+  uint8_t *flags;    // left/right flag per working id (capacity rounded up to 32)
+  uint32_t *bits;    // the flags packed, one bit per working id
+  uint32_t flag_words;  // 32-bit words of bits
   uint32_t *loc;     // working id -> local index inside a subtree block
   uint32_t *cnt;     // per-tile left counts of the device-wide levels
   uint32_t *tieflag; // per-segment "equal keys straddle the median"
