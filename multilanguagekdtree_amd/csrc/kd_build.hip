@@ -168,12 +168,154 @@ __global__ void k_shape(uint32_t *seg_start, uint32_t *seg_count, const SysDesc 
 // sort keys
 // ---------------------------------------------------------------------------
 
-__global__ void k_make_keys(const double *__restrict__ p, uint64_t *__restrict__ keys,
-                            uint32_t *__restrict__ ids, uint32_t n) {
-  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+// The id lists are sorted by the upper 32 bits of the ordered 64-bit image
+// of each coordinate (sign, exponent and 20 mantissa bits: 2^-20 relative
+// resolution in every binade).  That key order never contradicts the
+// coordinate order; bodies whose distinct coordinates share a key are put
+// in coordinate order afterwards (k_fix_runs, k_fix_long).  Equal
+// coordinates keep the sort's stable (id) order.
+
+__global__ void k_make_keys32(const double *__restrict__ p, uint32_t *__restrict__ keys,
+                              uint32_t *__restrict__ ids, uint32_t n, uint32_t *runs) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j == 0) runs[0] = 0;
   if (j >= n) return;
-  keys[j] = ordkey(p[j]);
+  keys[j] = (uint32_t)(ordkey(p[j]) >> 32);
   ids[j] = j;
+}
+
+// Runs of equal keys: a run of at most FIX_SHORT ids is put in (coordinate,
+// id) order by its first thread; longer runs whose coordinates are not all
+// equal are queued for k_fix_long (at most n/FIX_SHORT of them).
+constexpr uint32_t FIX_SHORT = 64;
+
+__device__ __forceinline__ bool coord_lt(const double *p, uint32_t a, uint32_t b) {
+  const double pa = p[a], pb = p[b];
+  return pa < pb || (pa == pb && a < b);
+}
+
+__global__ void k_fix_runs(const uint32_t *__restrict__ keys, uint32_t *__restrict__ ids,
+                           const double *__restrict__ p, uint32_t n, uint32_t *runs) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= n) return;
+  const uint32_t k = keys[i];
+  if (keys[i + 1] != k || (i > 0 && keys[i - 1] == k)) return;  // not the start of a run
+  uint32_t j = i + 1;
+  bool distinct = false;
+  const double p0 = p[ids[i]];
+  while (j < n && keys[j] == k) {
+    distinct |= p[ids[j]] != p0;
+    ++j;
+  }
+  if (!distinct) return;  // equal coordinates: the stable order stands
+  const uint32_t len = j - i;
+  if (len <= FIX_SHORT) {
+    for (uint32_t a = i + 1; a < j; ++a) {
+      const uint32_t v = ids[a];
+      uint32_t b = a;
+      while (b > i && coord_lt(p, v, ids[b - 1])) {
+        ids[b] = ids[b - 1];
+        --b;
+      }
+      ids[b] = v;
+    }
+    return;
+  }
+  const uint32_t slot = atomicAdd(&runs[0], 1u);
+  runs[1 + 2 * slot] = i;
+  runs[2 + 2 * slot] = len;
+}
+
+// Long runs, one workgroup each: chunks of LONG_LDS ids are bitonic-sorted in
+// LDS, then merged pairwise in global memory (merge path: each thread finds
+// the split of its output slice by binary search), ping-ponging between the
+// id list and a scratch list at the same offsets.  Runs are disjoint, so
+// workgroups never share memory.
+constexpr uint32_t LONG_LDS = 4096;
+
+__device__ void lds_bitonic(double *sk, uint32_t *si, uint32_t m) {
+  for (uint32_t size = 2; size <= m; size <<= 1) {
+    for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+        const uint32_t u = t ^ stride;
+        if (u > t) {
+          const bool up = (t & size) == 0;
+          const bool gt = sk[t] > sk[u] || (sk[t] == sk[u] && si[t] > si[u]);
+          if (gt == up) {
+            const double tk = sk[t];
+            sk[t] = sk[u];
+            sk[u] = tk;
+            const uint32_t ti = si[t];
+            si[t] = si[u];
+            si[u] = ti;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Sort ids[0..len) of one chunk (len <= LONG_LDS) in LDS.
+__device__ void sort_chunk(uint32_t *ids, uint32_t len, const double *__restrict__ p, double *sk,
+                           uint32_t *si) {
+  uint32_t m = 1;
+  while (m < len) m <<= 1;
+  for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+    if (t < len) {
+      si[t] = ids[t];
+      sk[t] = p[si[t]];
+    } else {
+      si[t] = 0xffffffffu;
+      sk[t] = __longlong_as_double(0x7ff0000000000000ll);  // +inf: padding sorts last
+    }
+  }
+  __syncthreads();
+  lds_bitonic(sk, si, m);
+  for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) ids[t] = si[t];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_fix_long(const uint32_t *runs, uint32_t *ids,
+                                                  uint32_t *scratch,
+                                                  const double *__restrict__ p) {
+  __shared__ double sk[LONG_LDS];
+  __shared__ uint32_t si[LONG_LDS];
+  const uint32_t nruns = runs[0];
+  for (uint32_t r = blockIdx.x; r < nruns; r += gridDim.x) {
+    const uint32_t st = runs[1 + 2 * r], len = runs[2 + 2 * r];
+    uint32_t *a = ids + st, *b = scratch + st;
+    for (uint32_t c = 0; c < len; c += LONG_LDS)
+      sort_chunk(a + c, min(LONG_LDS, len - c), p, sk, si);
+    for (uint32_t w = LONG_LDS; w < len; w <<= 1) {
+      // merge [lo, lo+w) and [lo+w, lo+2w) of a into b
+      for (uint32_t lo = 0; lo < len; lo += 2 * w) {
+        const uint32_t mid = min(lo + w, len), hi = min(lo + 2 * w, len);
+        const uint32_t na = mid - lo, nb = hi - mid, tot = na + nb;
+        const uint32_t per = (tot + blockDim.x - 1) / blockDim.x;
+        const uint32_t o0 = min(threadIdx.x * per, tot), o1 = min(o0 + per, tot);
+        // merge path: the first o0 outputs are a[lo, lo+i) and a[mid, mid+o0-i)
+        uint32_t l = o0 > nb ? o0 - nb : 0, h = min(o0, na);
+        while (l < h) {
+          const uint32_t i = (l + h) / 2, j = o0 - i - 1;
+          if (coord_lt(p, a[mid + j], a[lo + i])) h = i; else l = i + 1;
+        }
+        uint32_t i = l, j = o0 - l;
+        for (uint32_t o = o0; o < o1; ++o) {
+          const bool take_b = j < nb && (i >= na || coord_lt(p, a[mid + j], a[lo + i]));
+          b[lo + o] = take_b ? a[mid + j++] : a[lo + i++];
+        }
+      }
+      __threadfence_block();
+      __syncthreads();
+      uint32_t *t = a;
+      a = b;
+      b = t;
+    }
+    if (a != ids + st)
+      for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) ids[st + t] = a[t];
+    __syncthreads();
+  }
 }
 
 // Forest: system of every id of a coordinate-sorted list (nsys for padding
@@ -940,8 +1082,8 @@ static int bits_for(uint32_t v) {
 
 size_t sort_temp_bytes(uint32_t n, int nsys) {
   size_t bytes = 0;
-  (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint64_t *)nullptr, (uint64_t *)nullptr,
-                                  (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 64);
+  (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                  (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 32);
   if (nsys > 1) {
     size_t b2 = 0;
     (void)rocprim::radix_sort_pairs(nullptr, b2, (uint32_t *)nullptr, (uint32_t *)nullptr,
@@ -987,12 +1129,16 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   const uint32_t nsort = f.nsys == 1 ? f.max_n : f.total;
   if (sort_temp_bytes(nsort, f.nsys) > b.sort_tmp_bytes) return hipErrorInvalidValue;
   for (int d = 0; d < 3; ++d) {
-    k_make_keys<<<(nsort + 255) / 256, 256, 0, s>>>(coord[d], b.keys_in, b.ids_in, nsort);
+    uint32_t *key32 = reinterpret_cast<uint32_t *>(b.keys_in);
+    uint32_t *key32_out = reinterpret_cast<uint32_t *>(b.keys_out);
+    k_make_keys32<<<(nsort + 255) / 256, 256, 0, s>>>(coord[d], key32, b.ids_in, nsort, b.runs);
     size_t bytes = b.sort_tmp_bytes;
     uint32_t *by_key = f.nsys == 1 ? b.L[d] : b.Lb[d];
-    e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, b.keys_in, b.keys_out, b.ids_in, by_key,
-                                  nsort, 0, 64, s);
+    e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, key32, key32_out, b.ids_in, by_key, nsort,
+                                  0, 32, s);
     if (e != hipSuccess) return e;
+    k_fix_runs<<<(nsort + 255) / 256, 256, 0, s>>>(key32_out, by_key, coord[d], nsort, b.runs);
+    k_fix_long<<<256, 256, 0, s>>>(b.runs, by_key, b.ids_in, coord[d]);
     if (f.nsys > 1) {
       uint32_t *skey = reinterpret_cast<uint32_t *>(b.keys_in);
       uint32_t *skey_out = reinterpret_cast<uint32_t *>(b.keys_out);
