@@ -77,6 +77,7 @@ struct kd_ctx {
   std::vector<SysDesc> hsys;
   SysDesc *dsys = nullptr;
   uint32_t *sob = nullptr;  // system of every 64 body positions
+  uint32_t extra_recs = 0;  // walk records built beyond the precision's own
 };
 
 template <typename T>
@@ -404,8 +405,17 @@ int kd_download_device(kd_ctx *c, double *x, double *y, double *z, double *vx, d
   return download_impl(c, out, hipMemcpyDeviceToDevice);
 }
 
+// The walk records a build must write: the precision's own, plus those a
+// reverse sweep reads.
+static uint32_t records_for(const kd_ctx *c) {
+  const int p = c->params.precision;
+  const uint32_t own = p == KD_PREC_FP64 ? REC_64 : (p == KD_PREC_MIXED ? REC_32 : REC_D);
+  return own | c->extra_recs;
+}
+
 static int build_impl(kd_ctx *c) {
   int nxt = c->cur ^ 1;
+  c->tree.recs = records_for(c);
   HIP_TRY(launch_build(c->forest, c->params.leaf_size, c->st[c->cur], c->st[nxt], c->tree, c->b,
                        c->pos64, c->pos32, c->params.theta, c->stream));
   c->cur = nxt;
@@ -621,7 +631,7 @@ struct HostTree {
   std::vector<uint32_t> start, count;
   std::vector<uint8_t> sd;
   std::vector<double> sv, size;
-  std::vector<NodeRec64> rec;
+  std::vector<double> mm, mx, my, mz;  // mass and mass-weighted position sums
   int leaf;
 };
 
@@ -642,10 +652,10 @@ static int64_t emit_compact(const HostTree &t, uint32_t h, int64_t cur, kd_tree_
   v->split_dim[cur] = t.sd[h];
   v->split_val[cur] = t.sv[h];
   v->size[cur] = t.size[h];
-  v->m[cur] = t.rec[h].m;
-  v->cmx[cur] = t.rec[h].cx;
-  v->cmy[cur] = t.rec[h].cy;
-  v->cmz[cur] = t.rec[h].cz;
+  v->m[cur] = t.mm[h];
+  v->cmx[cur] = t.mx[h] / t.mm[h];
+  v->cmy[cur] = t.my[h] / t.mm[h];
+  v->cmz[cur] = t.mz[h] / t.mm[h];
   v->left[cur] = cur + 1;
   v->right[cur] = left + 1;
   return right;
@@ -672,10 +682,10 @@ static bool emit_padded(const HostTree &t, uint32_t h, int64_t cur, int64_t limi
   v->split_dim[cur] = t.sd[h];
   v->split_val[cur] = t.sv[h];
   v->size[cur] = t.size[h];
-  v->m[cur] = t.rec[h].m;
-  v->cmx[cur] = t.rec[h].cx;
-  v->cmy[cur] = t.rec[h].cy;
-  v->cmz[cur] = t.rec[h].cz;
+  v->m[cur] = t.mm[h];
+  v->cmx[cur] = t.mx[h] / t.mm[h];
+  v->cmy[cur] = t.my[h] / t.mm[h];
+  v->cmz[cur] = t.mz[h] / t.mm[h];
   v->left[cur] = cur + 1;
   v->right[cur] = cur + 1 + nn;
   return true;
@@ -717,7 +727,10 @@ static int export_impl(kd_ctx *c, const SysDesc &d, int32_t layout, kd_tree_view
   t.sd.resize(slots);
   t.sv.resize(slots);
   t.size.resize(slots);
-  t.rec.resize(slots);
+  t.mm.resize(slots);
+  t.mx.resize(slots);
+  t.my.resize(slots);
+  t.mz.resize(slots);
   std::vector<uint32_t> oid(n);
   hipStream_t s = c->stream;
   HIP_TRY(hipMemcpyAsync(t.start.data(), tr.seg_start, 4 * slots, hipMemcpyDeviceToHost, s));
@@ -725,8 +738,10 @@ static int export_impl(kd_ctx *c, const SysDesc &d, int32_t layout, kd_tree_view
   HIP_TRY(hipMemcpyAsync(t.sd.data(), tr.split_dim, slots, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(t.sv.data(), tr.split_val, 8 * slots, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(t.size.data(), tr.size, 8 * slots, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(t.rec.data(), tr.rec64, sizeof(NodeRec64) * slots,
-                         hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.mm.data(), tr.mm, 8 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.mx.data(), tr.mx, 8 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.my.data(), tr.my, 8 * slots, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(t.mz.data(), tr.mz, 8 * slots, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipMemcpyAsync(oid.data(), c->st[c->cur].oid + d.body_off, 4 * n,
                          hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -1407,6 +1422,7 @@ extern "C" int kd_adj_create(int64_t n, int64_t steps, int64_t every, const kd_p
     g_err = msg;
     return rc;
   }
+  a->c->extra_recs = REC_64 | REC_32;  // the reverse sweep reads both
   a->n = n;
   a->steps = steps;
   a->every = std::min(every, steps);

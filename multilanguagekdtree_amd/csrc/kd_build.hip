@@ -650,13 +650,13 @@ __device__ void write_leaf_rec(const TreeDev &t, uint32_t h, uint32_t start,
   r.skip = skip;
   r.pad0 = 0;
   r.pad1 = 0.0;
-  t.rec64[h] = r;
+  if (t.recs & REC_64) t.rec64[h] = r;
   NodeRec32 q;
   q.cx = q.cy = q.cz = q.m = q.size2 = 0.f;
   q.start = start;
   q.count = count;
   q.skip = skip;
-  t.rec32[h] = q;
+  if (t.recs & REC_32) t.rec32[h] = q;
   NodeRecD dd;
   dd.cx = dd.cy = dd.cz = 0.0;
   dd.m = dd.size2 = 0.f;
@@ -665,7 +665,7 @@ __device__ void write_leaf_rec(const TreeDev &t, uint32_t h, uint32_t start,
   dd.skip = skip;
   dd.pad0 = 0;
   dd.pad1 = 0.0;
-  t.recd[h] = dd;
+  if (t.recs & REC_D) t.recd[h] = dd;
 }
 
 __device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
@@ -688,7 +688,7 @@ __device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
   r.skip = skip;
   r.pad0 = 0;
   r.pad1 = 0.0;
-  t.rec64[h] = r;
+  if (t.recs & REC_64) t.rec64[h] = r;
   NodeRec32 q;
   q.cx = (float)cx;
   q.cy = (float)cy;
@@ -698,7 +698,7 @@ __device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
   q.start = start;
   q.count = count;
   q.skip = skip;
-  t.rec32[h] = q;
+  if (t.recs & REC_32) t.rec32[h] = q;
   NodeRecD dd;
   dd.cx = cx;
   dd.cy = cy;
@@ -710,7 +710,7 @@ __device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
   dd.skip = skip;
   dd.pad0 = 0;
   dd.pad1 = 0.0;
-  t.recd[h] = dd;
+  if (t.recs & REC_D) t.recd[h] = dd;
 }
 
 __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {

@@ -67,7 +67,10 @@ struct TreeDev {
   NodeRec64 *rec64;
   NodeRec32 *rec32;
   NodeRecD *recd;
+  // which walk records the build writes: REC_64 | REC_32 | REC_D
+  uint32_t recs = 7;
 };
+constexpr uint32_t REC_64 = 1, REC_32 = 2, REC_D = 4;
 
 // The same tree with every per-node array advanced by o slots.
 __host__ __device__ inline TreeDev tree_at(const TreeDev &t, uint32_t o) {
