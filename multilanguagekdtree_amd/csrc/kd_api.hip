@@ -129,6 +129,7 @@ static int alloc_nodes(kd_ctx *c, int64_t s) {
     ALLOC(c->node_allocs, t.rec64, s);
     ALLOC(c->node_allocs, t.rec32, s);
     ALLOC(c->node_allocs, t.recd, s);
+    ALLOC(c->node_allocs, t.lmask, s);
     c->slots_alloc = s;
   }
   return KD_OK;

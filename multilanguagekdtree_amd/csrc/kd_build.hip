@@ -134,13 +134,14 @@ __device__ __forceinline__ SegV block_seg_carry(SegV agg, uint64_t *shv,
 
 // Segment of every heap slot: descend from the root splitting at floor(c/2);
 // slots below a leaf get count 0.
-__global__ void k_shape(uint32_t *seg_start, uint32_t *seg_count, const SysDesc *sys,
-                        int leaf) {
+__global__ void k_shape(uint32_t *seg_start, uint32_t *seg_count, uint8_t *lmask,
+                        const SysDesc *sys, int leaf) {
   const SysDesc S = sys[blockIdx.y];
   int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= S.slots) return;
   seg_start += S.node_off;
   seg_count += S.node_off;
+  lmask[S.node_off + h] = 0;  // every list of a root starts in the first buffer
   const uint32_t n = S.n;
   uint32_t hp = (uint32_t)h + 1;
   int lev = 31 - __clz(hp);
@@ -342,7 +343,17 @@ struct Coords {
   const double *c[3];
 };
 
-__global__ void k_level_split(Lists L, Coords P, TreeDev t0, const SysDesc *sys,
+// The two buffers of the three sorted lists.  A node's lists live in A or B
+// per its mask (bit d: list d in B); splitting a node moves the two lists
+// that are partitioned to the other buffer and leaves the split-dimension
+// list where it is (it is already partitioned at the median).
+struct ListBufs {
+  Lists A, B;
+  __device__ uint32_t *in(uint32_t m, int d) const { return ((m >> d) & 1u) ? B.l[d] : A.l[d]; }
+  __device__ uint32_t *out(uint32_t m, int d) const { return ((m >> d) & 1u) ? A.l[d] : B.l[d]; }
+};
+
+__global__ void k_level_split(ListBufs LB, Coords P, TreeDev t0, const SysDesc *sys,
                               uint32_t level, uint32_t *tieflag, uint32_t tf_stride) {
   const SysDesc S = sys[blockIdx.y];
   uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -352,19 +363,24 @@ __global__ void k_level_split(Lists L, Coords P, TreeDev t0, const SysDesc *sys,
   tieflag += blockIdx.y * tf_stride;
   uint32_t h = nseg - 1 + k;
   uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
+  const uint32_t m = t.lmask[h];
   double ext[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double lo = P.c[d][L.l[d][s]];
-    double hi = P.c[d][L.l[d][s + c - 1]];
+    const uint32_t *ld = LB.in(m, d);
+    double lo = P.c[d][ld[s]];
+    double hi = P.c[d][ld[s + c - 1]];
     ext[d] = hi - lo;
   }
   int sd = 0;
   if (ext[1] > ext[sd]) sd = 1;
   if (ext[2] > ext[sd]) sd = 2;
   uint32_t mid = s + c / 2;
-  double kv = P.c[sd][L.l[sd][mid]];
-  double kp = P.c[sd][L.l[sd][mid - 1]];
+  double kv = P.c[sd][LB.in(m, sd)[mid]];
+  double kp = P.c[sd][LB.in(m, sd)[mid - 1]];
+  const uint8_t cm = (uint8_t)(m ^ (7u ^ (1u << sd)));
+  t.lmask[2 * h + 1] = cm;
+  t.lmask[2 * h + 2] = cm;
   t.split_dim[h] = (uint8_t)sd;
   t.split_val[h] = kv;
   t.size[h] = ext[sd];
@@ -375,7 +391,7 @@ __global__ void k_level_split(Lists L, Coords P, TreeDev t0, const SysDesc *sys,
 // are those with the smallest original indices.  One 1024-thread block per
 // flagged segment: radix-select the threshold index, then stably partition
 // the run (scratch = the same list's output buffer, overwritten later).
-__global__ __launch_bounds__(1024) void k_level_tiefix(Lists L, Lists scratch,
+__global__ __launch_bounds__(1024) void k_level_tiefix(ListBufs LB,
                                                        Coords P, TreeDev t0,
                                                        const SysDesc *sys,
                                                        const uint32_t *oid,
@@ -390,8 +406,8 @@ __global__ __launch_bounds__(1024) void k_level_tiefix(Lists L, Lists scratch,
   uint32_t h = (1u << level) - 1 + k;
   uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
-  uint32_t *ls = L.l[sd];
-  uint32_t *tmp = scratch.l[sd];
+  uint32_t *ls = LB.in(t.lmask[h], sd);
+  uint32_t *tmp = LB.out(t.lmask[h], sd);  // not live for this node: scratch
   const double *key = P.c[sd];
   uint32_t mid = s + c / 2;
   double kv = key[ls[mid]];
@@ -463,7 +479,7 @@ __global__ __launch_bounds__(1024) void k_level_tiefix(Lists L, Lists scratch,
   for (uint32_t p = r0 + threadIdx.x; p < r1; p += NT) ls[p] = tmp[p];
 }
 
-__global__ __launch_bounds__(LVL_THREADS) void k_level_flags(Lists L, TreeDev t0,
+__global__ __launch_bounds__(LVL_THREADS) void k_level_flags(ListBufs LB, TreeDev t0,
                                                              const SysDesc *sys,
                                                              uint32_t level,
                                                              uint32_t T,
@@ -476,7 +492,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_flags(Lists L, TreeDev t0
   uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
   uint32_t mid = s + c / 2, e = s + c;
-  const uint32_t *ls = L.l[sd];
+  const uint32_t *ls = LB.in(t.lmask[h], sd);
   uint32_t base = s + q * LVL_TILE;
   for (int i = 0; i < LVL_ITEMS; ++i) {
     uint32_t j = base + i * LVL_THREADS + threadIdx.x;
@@ -508,7 +524,7 @@ __device__ __forceinline__ uint32_t flag_bit(const uint32_t *__restrict__ bits, 
   return (bits[id >> 5] >> (id & 31u)) & 1u;
 }
 
-__global__ __launch_bounds__(LVL_THREADS) void k_level_count(Lists L, TreeDev t0,
+__global__ __launch_bounds__(LVL_THREADS) void k_level_count(ListBufs LB, TreeDev t0,
                                                              const SysDesc *sys,
                                                              uint32_t level,
                                                              uint32_t T,
@@ -529,7 +545,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_count(Lists L, TreeDev t0
   for (int d = 0; d < 3; ++d) {
     uint32_t local = 0;
     if (d != sd) {
-      const uint32_t *ld = L.l[d];
+      const uint32_t *ld = LB.in(t.lmask[h], d);
       for (int i = 0; i < LVL_ITEMS; ++i) {
         uint32_t j = base + i * LVL_THREADS + threadIdx.x;
         if (j < e) local += flag_bit(bits, ld[j]);
@@ -559,7 +575,7 @@ __global__ __launch_bounds__(256) void k_level_scan(uint32_t *cnt, uint32_t T,
   }
 }
 
-__global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(Lists L, Lists Lout,
+__global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(ListBufs LB,
                                                                TreeDev t0,
                                                                const SysDesc *sys,
                                                                uint32_t level,
@@ -578,16 +594,11 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(Lists L, Lists Lo
   int sd = t.split_dim[h];
   uint32_t mid = s + c / 2, e = s + c;
   uint32_t base = s + q * LVL_TILE + threadIdx.x * LVL_ITEMS;
+  const uint32_t m = t.lmask[h];
   for (int d = 0; d < 3; ++d) {
-    const uint32_t *ld = L.l[d];
-    uint32_t *od = Lout.l[d];
-    if (d == sd) {
-      for (int i = 0; i < LVL_ITEMS; ++i) {
-        uint32_t j = s + q * LVL_TILE + i * LVL_THREADS + threadIdx.x;
-        if (j < e) od[j] = ld[j];
-      }
-      continue;
-    }
+    if (d == sd) continue;  // already partitioned at the median: stays in place
+    const uint32_t *ld = LB.in(m, d);
+    uint32_t *od = LB.out(m, d);
     uint32_t id[LVL_ITEMS];
     uint32_t f[LVL_ITEMS];
     uint32_t local = 0;
@@ -618,7 +629,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(Lists L, Lists Lo
 // ---------------------------------------------------------------------------
 
 struct SubArgs {
-  Lists L[2];       // sorted lists after an even / odd number of device-wide levels
+  ListBufs LB;      // sorted lists; a node's buffers per its mask
   const SysDesc *sys;
   Coords P;         // positions, working order
   const double *m;  // masses, working order
@@ -728,7 +739,8 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   const uint32_t level = S.sub_level;
   if (k >= (1u << level)) return;
   const TreeDev t = tree_at(a.t, S.node_off);
-  const Lists L = a.L[level & 1];
+  const uint32_t m0 = t.lmask[(1u << level) - 1 + k];
+  const uint32_t *L0 = a.LB.in(m0, 0), *L1 = a.LB.in(m0, 1), *L2 = a.LB.in(m0, 2);
   const uint32_t h0 = (1u << level) - 1 + k;
   // s0l: first body of the segment within its system; s0: in the body
   // arrays (tree order); sl: in the sorted lists
@@ -739,7 +751,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   const int tid = threadIdx.x;
 
   for (uint32_t i = tid; i < n0; i += NT) {
-    uint32_t id = L.l[0][sl + i];
+    uint32_t id = L0[sl + i];
     ids[i] = id;
     lst[0][0][i] = (uint16_t)i;
     a.loc[id] = i;
@@ -747,8 +759,8 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   __threadfence_block();
   __syncthreads();
   for (uint32_t i = tid; i < n0; i += NT) {
-    lst[0][1][i] = (uint16_t)a.loc[L.l[1][sl + i]];
-    lst[0][2][i] = (uint16_t)a.loc[L.l[2][sl + i]];
+    lst[0][1][i] = (uint16_t)a.loc[L1[sl + i]];
+    lst[0][2][i] = (uint16_t)a.loc[L2[sl + i]];
   }
   __syncthreads();
 
@@ -1112,8 +1124,8 @@ size_t tieflag_entries(const Forest &f) { return (size_t)f.nsys << f.max_sub_lev
 
 hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t s) {
   const unsigned blocks = (f.max_slots + 255) / 256;
-  k_shape<<<dim3(blocks, (unsigned)f.nsys), dim3(256), 0, s>>>(t.seg_start, t.seg_count, f.sys,
-                                                               leaf);
+  k_shape<<<dim3(blocks, (unsigned)f.nsys), dim3(256), 0, s>>>(t.seg_start, t.seg_count,
+                                                               t.lmask, f.sys, leaf);
   return hipGetLastError();
 }
 
@@ -1150,32 +1162,28 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
       if (e != hipSuccess) return e;
     }
   }
-  Lists L{{b.L[0], b.L[1], b.L[2]}};
-  Lists Lb{{b.Lb[0], b.Lb[1], b.Lb[2]}};
-  const Lists L0 = L, L1 = Lb;
+  ListBufs LB;
+  LB.A = Lists{{b.L[0], b.L[1], b.L[2]}};
+  LB.B = Lists{{b.Lb[0], b.Lb[1], b.Lb[2]}};
   Coords P{{in.x, in.y, in.z}};
   const uint32_t tf_stride = 1u << f.max_sub_level;
   for (int lev = 0; lev < f.max_sub_level; ++lev) {
     const uint32_t nseg = 1u << lev;
     const uint32_t T = level_tiles(f, lev);
-    k_level_split<<<dim3((nseg + 255) / 256, ny), 256, 0, s>>>(L, P, t, f.sys, lev, b.tieflag,
+    k_level_split<<<dim3((nseg + 255) / 256, ny), 256, 0, s>>>(LB, P, t, f.sys, lev, b.tieflag,
                                                                 tf_stride);
-    k_level_tiefix<<<dim3(nseg, ny), 1024, 0, s>>>(L, Lb, P, t, f.sys, in.oid, lev, b.tieflag,
+    k_level_tiefix<<<dim3(nseg, ny), 1024, 0, s>>>(LB, P, t, f.sys, in.oid, lev, b.tieflag,
                                                    tf_stride);
-    k_level_flags<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, t, f.sys, lev, T, b.flags);
+    k_level_flags<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.flags);
     k_level_pack<<<(b.flag_words + 255) / 256, 256, 0, s>>>(b.flags, b.bits, b.flag_words);
-    k_level_count<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, t, f.sys, lev, T, b.bits,
+    k_level_count<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.bits,
                                                              b.cnt);
     k_level_scan<<<dim3(3 * nseg, ny), 256, 0, s>>>(b.cnt, T, f.sys, lev);
-    k_level_scatter<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(L, Lb, t, f.sys, lev, T, b.bits,
+    k_level_scatter<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.bits,
                                                                b.cnt);
-    Lists tmp = L;
-    L = Lb;
-    Lb = tmp;
   }
   SubArgs a;
-  a.L[0] = L0;
-  a.L[1] = L1;
+  a.LB = LB;
   a.sys = f.sys;
   a.P = P;
   a.m = in.m;

@@ -67,6 +67,8 @@ struct TreeDev {
   NodeRec64 *rec64;
   NodeRec32 *rec32;
   NodeRecD *recd;
+  // build: bit d set iff the node's sorted list d sits in the second buffer
+  uint8_t *lmask;
   // which walk records the build writes: REC_64 | REC_32 | REC_D
   uint32_t recs = 7;
 };
@@ -87,6 +89,7 @@ __host__ __device__ inline TreeDev tree_at(const TreeDev &t, uint32_t o) {
   r.rec64 += o;
   r.rec32 += o;
   r.recd += o;
+  r.lmask += o;
   return r;
 }
 
