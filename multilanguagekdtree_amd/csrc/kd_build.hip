@@ -963,23 +963,25 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
     a.out.vz[tpos] = a.in.vz[w];
     a.out.m[tpos] = m;
     a.out.oid[tpos] = a.in.oid[w];
-    a.pos64[tpos] = make_double4(x, y, z, m);
-    a.pos32[tpos] = make_float4((float)x, (float)y, (float)z, (float)m);
+    // the walk's body records, for the arithmetic that reads them
+    if (t.recs & (REC_64 | REC_D)) a.pos64[tpos] = make_double4(x, y, z, m);
+    if (t.recs & REC_32) a.pos32[tpos] = make_float4((float)x, (float)y, (float)z, (float)m);
   }
   __threadfence_block();
   __syncthreads();
-  // leaf moments, summed in canonical order from the tree-ordered records
+  // leaf moments, summed in canonical order from the tree-ordered state
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     uint32_t j = tid * IT + it;
     if (j >= n0 || j != st[it]) continue;
     double mm = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
     for (uint32_t q = 0; q < ct[it]; ++q) {
-      const double4 b = a.pos64[s0 + st[it] + q];
-      mm += b.w;
-      mx += b.w * b.x;
-      my += b.w * b.y;
-      mz += b.w * b.z;
+      const uint32_t tp = s0 + st[it] + q;
+      const double bm = a.out.m[tp];
+      mm += bm;
+      mx += bm * a.out.x[tp];
+      my += bm * a.out.y[tp];
+      mz += bm * a.out.z[tp];
     }
     const uint32_t h = hl[it];
     t.mm[h] = mm;
