@@ -169,19 +169,69 @@ __global__ void k_shape(uint32_t *seg_start, uint32_t *seg_count, uint8_t *lmask
 // sort keys
 // ---------------------------------------------------------------------------
 
-// The id lists are sorted by the upper 32 bits of the ordered 64-bit image
-// of each coordinate (sign, exponent and 20 mantissa bits: 2^-20 relative
-// resolution in every binade).  That key order never contradicts the
-// coordinate order; bodies whose distinct coordinates share a key are put
-// in coordinate order afterwards (k_fix_runs, k_fix_long).  Equal
-// coordinates keep the sort's stable (id) order.
+// The id lists are sorted by a 32-bit fixed-point image of each coordinate
+// over its current range, key = floor((x - min) (2^32 - 1) / (max - min))
+// (computed on halves, so no range overflows).  Every rounding step of the
+// map is monotone, so the key order never contradicts the coordinate order;
+// bodies whose distinct coordinates share a key are put in coordinate order
+// afterwards (k_fix_runs, k_fix_long).  Equal coordinates keep the sort's
+// stable (id) order.
 
-__global__ void k_make_keys32(const double *__restrict__ p, uint32_t *__restrict__ keys,
-                              uint32_t *__restrict__ ids, uint32_t n, uint32_t *runs) {
+__device__ __forceinline__ double ordkey_inv(uint64_t k) {
+  const uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
+// range[2d], range[2d+1]: ordered-key minimum and maximum of dimension d.
+__global__ void k_range_init(unsigned long long *range) {
+  const int t = threadIdx.x;
+  if (t < 3) {
+    range[2 * t] = ~0ull;
+    range[2 * t + 1] = 0ull;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_range(const double *__restrict__ x,
+                                               const double *__restrict__ y,
+                                               const double *__restrict__ z, uint32_t n,
+                                               unsigned long long *range) {
+  uint64_t lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint64_t k[3] = {ordkey(x[i]), ordkey(y[i]), ordkey(z[i])};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = k[d] < lo[d] ? k[d] : lo[d];
+      hi[d] = k[d] > hi[d] ? k[d] : hi[d];
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t a = __shfl_xor(lo[d], o, 64), b = __shfl_xor(hi[d], o, 64);
+      lo[d] = a < lo[d] ? a : lo[d];
+      hi[d] = b > hi[d] ? b : hi[d];
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&range[2 * d], (unsigned long long)lo[d]);
+      atomicMax(&range[2 * d + 1], (unsigned long long)hi[d]);
+    }
+  }
+}
+
+__global__ void k_make_keys32(const double *__restrict__ p, const unsigned long long *range,
+                              uint32_t *__restrict__ keys, uint32_t *__restrict__ ids, uint32_t n,
+                              uint32_t *runs) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j == 0) runs[0] = 0;
   if (j >= n) return;
-  keys[j] = (uint32_t)(ordkey(p[j]) >> 32);
+  const double lo = 0.5 * ordkey_inv(range[0]), hi = 0.5 * ordkey_inv(range[1]);
+  const double scale = hi > lo ? 4294967295.0 / (hi - lo) : 0.0;
+  const double v = 0.5 * p[j];
+  keys[j] = (uint32_t)fmin(fmax((v - lo) * scale, 0.0), 4294967295.0);
   ids[j] = j;
 }
 
@@ -1142,10 +1192,14 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   // in coordinate order at [list_off, list_off + n) (padding last)
   const uint32_t nsort = f.nsys == 1 ? f.max_n : f.total;
   if (sort_temp_bytes(nsort, f.nsys) > b.sort_tmp_bytes) return hipErrorInvalidValue;
+  k_range_init<<<1, 64, 0, s>>>(b.range);
+  k_range<<<std::min<uint32_t>((nsort + 255) / 256, 2048), 256, 0, s>>>(in.x, in.y, in.z, nsort,
+                                                                         b.range);
   for (int d = 0; d < 3; ++d) {
     uint32_t *key32 = reinterpret_cast<uint32_t *>(b.keys_in);
     uint32_t *key32_out = reinterpret_cast<uint32_t *>(b.keys_out);
-    k_make_keys32<<<(nsort + 255) / 256, 256, 0, s>>>(coord[d], key32, b.ids_in, nsort, b.runs);
+    k_make_keys32<<<(nsort + 255) / 256, 256, 0, s>>>(coord[d], b.range + 2 * d, key32, b.ids_in,
+                                                      nsort, b.runs);
     size_t bytes = b.sort_tmp_bytes;
     uint32_t *by_key = f.nsys == 1 ? b.L[d] : b.Lb[d];
     e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, key32, key32_out, b.ids_in, by_key, nsort,

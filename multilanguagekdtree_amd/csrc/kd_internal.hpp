@@ -139,6 +139,7 @@ struct BuildBufs {
   uint32_t *L[3], *Lb[3];  // sorted id lists and their partition buffers
   uint64_t *keys_in, *keys_out;
   uint32_t *ids_in;
+  unsigned long long *range;  // ordered-key minimum and maximum of x, y, z
   uint32_t *runs;             // long runs of equal sort keys: count, then (start, length)
   void *sort_tmp;
   size_t sort_tmp_bytes;

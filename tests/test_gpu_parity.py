@@ -146,17 +146,18 @@ def test_lattice_ties(orc, shape):
 
 @pytest.mark.parametrize("cluster", [40, 3000, 9000])
 def test_coordinates_sharing_sort_keys(orc, cluster):
-    # distinct coordinates within 2^-20 relative of each other share their
-    # 32-bit sort key; runs of such keys are ordered by the exact coordinate
-    # (short runs in place, up to 4096 in LDS, longer ones by a merge sort)
+    # distinct coordinates closer than 2^-32 of the coordinate range share
+    # their 32-bit sort key; runs of such keys are ordered by the exact
+    # coordinate (short runs in place, up to 4096 in LDS, longer ones by a
+    # merge sort)
     rng = np.random.default_rng(cluster)
     n = 20000
     x = rng.uniform(-2.0, 2.0, n)
     y = rng.uniform(-2.0, 2.0, n)
     z = rng.normal(0.0, 1e-3, n)
     k = rng.permutation(cluster)
-    x[:cluster] = 1.0 + k * 2.0 ** -40
-    y[cluster:2 * cluster] = -0.75 - k * 2.0 ** -41
+    x[:cluster] = 1.0 + k * 2.0 ** -50
+    y[cluster:2 * cluster] = -0.75 - k * 2.0 ** -51
     z[:cluster] = 1e-3 * (1.0 + rng.permutation(cluster) * 2.0 ** -45)
     b = dict(x=x, y=y, z=z, vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n),
              m=np.full(n, 1.0 / n))
