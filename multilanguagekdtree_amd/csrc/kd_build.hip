@@ -245,12 +245,11 @@ __device__ __forceinline__ bool coord_lt(const double *p, uint32_t a, uint32_t b
   return pa < pb || (pa == pb && a < b);
 }
 
-__global__ void k_fix_runs(const uint32_t *__restrict__ keys, uint32_t *__restrict__ ids,
-                           const double *__restrict__ p, uint32_t n, uint32_t *runs) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i + 1 >= n) return;
+// The run starting at i (keys[i] == keys[i + 1], and i is 0 or keys[i - 1]
+// differs), fixed by one thread.
+__device__ void fix_run(const uint32_t *__restrict__ keys, uint32_t *__restrict__ ids,
+                        const double *__restrict__ p, uint32_t n, uint32_t *runs, uint32_t i) {
   const uint32_t k = keys[i];
-  if (keys[i + 1] != k || (i > 0 && keys[i - 1] == k)) return;  // not the start of a run
   uint32_t j = i + 1;
   bool distinct = false;
   const double p0 = p[ids[i]];
@@ -275,6 +274,44 @@ __global__ void k_fix_runs(const uint32_t *__restrict__ keys, uint32_t *__restri
   const uint32_t slot = atomicAdd(&runs[0], 1u);
   runs[1 + 2 * slot] = i;
   runs[2 + 2 * slot] = len;
+}
+
+// Each thread scans FIX_PER consecutive sorted keys (two 16-byte loads plus
+// one key either side) for run starts; the few it finds are fixed serially.
+// Runs are disjoint, so threads never touch the same ids.
+constexpr uint32_t FIX_PER = 8;
+
+__global__ __launch_bounds__(256) void k_fix_runs(const uint32_t *__restrict__ keys,
+                                                  uint32_t *__restrict__ ids,
+                                                  const double *__restrict__ p, uint32_t n,
+                                                  uint32_t *runs) {
+  const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * FIX_PER;
+  if (i0 >= n) return;
+  uint32_t k[FIX_PER + 2];  // k[0] = keys[i0 - 1], k[1 + j] = keys[i0 + j]
+  const uint32_t m = min(FIX_PER, n - i0);
+  if (m == FIX_PER) {
+    const uint4 a = *reinterpret_cast<const uint4 *>(keys + i0);
+    const uint4 b = *reinterpret_cast<const uint4 *>(keys + i0 + 4);
+    k[1] = a.x, k[2] = a.y, k[3] = a.z, k[4] = a.w;
+    k[5] = b.x, k[6] = b.y, k[7] = b.z, k[8] = b.w;
+  } else {
+#pragma unroll
+    for (uint32_t j = 0; j < FIX_PER; ++j) k[1 + j] = j < m ? keys[i0 + j] : 0u;
+  }
+  k[0] = i0 > 0 ? keys[i0 - 1] : 0u;
+  k[FIX_PER + 1] = i0 + FIX_PER < n ? keys[i0 + FIX_PER] : 0u;
+  uint32_t starts = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < FIX_PER; ++j) {
+    const bool next_eq = i0 + j + 1 < n && k[2 + j] == k[1 + j];
+    const bool prev_eq = i0 + j > 0 && k[j] == k[1 + j];
+    starts |= (next_eq && !prev_eq) ? (1u << j) : 0u;
+  }
+  while (starts) {
+    const uint32_t j = __builtin_ctz(starts);
+    starts &= starts - 1;
+    fix_run(keys, ids, p, n, runs, i0 + j);
+  }
 }
 
 // Long runs, one workgroup each: chunks of LONG_LDS ids are bitonic-sorted in
@@ -1205,7 +1242,8 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
     e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, key32, key32_out, b.ids_in, by_key, nsort,
                                   0, 32, s);
     if (e != hipSuccess) return e;
-    k_fix_runs<<<(nsort + 255) / 256, 256, 0, s>>>(key32_out, by_key, coord[d], nsort, b.runs);
+    k_fix_runs<<<(nsort + 256 * FIX_PER - 1) / (256 * FIX_PER), 256, 0, s>>>(
+        key32_out, by_key, coord[d], nsort, b.runs);
     k_fix_long<<<256, 256, 0, s>>>(b.runs, by_key, b.ids_in, coord[d]);
     if (f.nsys > 1) {
       uint32_t *skey = reinterpret_cast<uint32_t *>(b.keys_in);

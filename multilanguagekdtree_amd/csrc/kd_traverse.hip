@@ -67,6 +67,7 @@ __device__ __forceinline__ uint32_t heap_next(uint32_t h, int &depth) {
 
 struct F64 {
   static constexpr int CHUNK = 4;  // leaf bodies per batch of scalar loads
+  static constexpr bool SELF_BY_INDEX = true;  // self pair dropped by a select
   using Rec = NodeRec64;
   using Body = double4;
   using Self = double4;
@@ -130,16 +131,26 @@ __device__ __forceinline__ void kahan_fold(KahanAcc &a) {
   }
 }
 
+// F32 pair terms take d2 + e2 with a per-body floor e2 (added inside the
+// first FMA, so it costs nothing).  The self pair (d = 0 exactly) then gives
+// finite g times 0 = 0 with no compare or select: e2 >= max(m, 1) 2^-84 keeps
+// m r^-3 below 2^126.  e2 >= (|p|_max 2^-30)^2 places the floor 64 times below
+// the FP32 resolution of the body's own coordinates, so distinct pairs move
+// by less than their coordinate rounding already moves them.
 struct F32 {
   static constexpr int CHUNK = 8;
+  static constexpr bool SELF_BY_INDEX = false;
   using Rec = NodeRec32;
   using Body = float4;
-  using Self = float4;
+  using Self = float4;  // x, y, z, e2
   using Acc = KahanAcc;
   struct Diff {
     float dx, dy, dz, d2;
   };
-  __device__ static Self self(const Body &b) { return b; }
+  __device__ static Self self(const Body &b) {
+    const float a = fmaxf(fabsf(b.x), fmaxf(fabsf(b.y), fabsf(b.z))) * 0x1p-30f;
+    return make_float4(b.x, b.y, b.z, fmaxf(a * a, fmaxf(b.w, 1.f) * 0x1p-84f));
+  }
   __device__ static bool test(const Self &p, const Rec &r, double, Diff &f) {
     f.dx = p.x - r.cx;
     f.dy = p.y - r.cy;
@@ -157,14 +168,14 @@ struct F32 {
     a.p[1] += g * f.dy;
     a.p[2] += g * f.dz;
   }
-  __device__ static void p2p(Acc &a, const Self &p, const Body &q, bool self_pair) {
+  __device__ static void p2p(Acc &a, const Self &p, const Body &q, bool skip) {
     float dx = p.x - q.x;
     float dy = p.y - q.y;
     float dz = p.z - q.z;
-    float d2 = dx * dx + dy * dy + dz * dz;
+    float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, p.w)));
     float ri = __builtin_amdgcn_rsqf(d2);
     float g = -q.w * (ri * ri * ri);
-    g = self_pair ? 0.f : g;
+    g = skip ? 0.f : g;  // padding of a predicated batch; constant false otherwise
     a.p[0] += g * dx;
     a.p[1] += g * dy;
     a.p[2] += g * dz;
@@ -177,6 +188,7 @@ struct F32 {
 
 struct D64 {
   static constexpr int CHUNK = 4;
+  static constexpr bool SELF_BY_INDEX = true;
   using Rec = NodeRecD;
   using Body = double4;
   using Self = double4;
@@ -252,14 +264,14 @@ __device__ __forceinline__ void leaf_batch(typename P::Acc &acc, const typename 
 #pragma unroll
     for (int j = 0; j < B; ++j) {
       const bool use = (uint32_t)j < n;
-      P::p2p(acc, p, q[j], !use || s0 + j == me);
+      P::p2p(acc, p, q[j], !use || (P::SELF_BY_INDEX && s0 + j == me));
       if (COUNT) np += (use && s0 + j != me) ? 1 : 0;
     }
     return;
   }
 #define KD_P2P(j)                                          \
   case (j) + 1:                                            \
-    P::p2p(acc, p, q[(j) % B], s0 + (j) == me);            \
+    P::p2p(acc, p, q[(j) % B], P::SELF_BY_INDEX && s0 + (j) == me); \
     if (COUNT) np += (s0 + (j) != me) ? 1 : 0;             \
     [[fallthrough]];
   switch (n) {
@@ -283,7 +295,9 @@ __device__ __forceinline__ void leaf_batch(typename P::Acc &acc, const typename 
 template <class P, bool COUNT, bool PRED>
 __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ nodes_m1,
                                           const typename P::Body *__restrict__ bodies,
-                                          const WalkArgs<P> &a, uint32_t wbase) {
+                                          const WalkArgs<P> &a, uint32_t wbase_in) {
+  // wave-uniform by construction; readfirstlane lets the compiler know it
+  const uint32_t wbase = uni(wbase_in);
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t me = wbase + lane;
   const bool valid = me < a.n;
@@ -295,7 +309,10 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
   do {
     const typename P::Rec r = load_rec(nodes_m1, uni(h));
     const int depth = 31 - __builtin_clz(h);
-    const bool active = depth <= D;
+    // the lanes taking part, as a scalar mask (kept on the scalar unit) and
+    // as the lane's own condition
+    const uint64_t actm = __builtin_amdgcn_ballot_w64(depth <= D);
+    const bool active = __builtin_amdgcn_inverse_ballot_w64(actm);
     if (COUNT) ++wv;
     if (r.count <= a.leaf) {
       if (COUNT) ++wl;
@@ -309,15 +326,18 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
       // the test runs on every lane (no divergence); only active lanes use it
       typename P::Diff f;
       const bool ok = P::test(p, r, a.theta2, f);
-      const bool take = active && ok;
-      const bool open = active && !ok;
-      P::m2p_sel(acc, r, f, take);
+      // lane masks combined on the scalar unit, then used as lane conditions
+      const uint64_t okm = __builtin_amdgcn_ballot_w64(ok);
+      const uint64_t om = actm & ~okm;
+      const bool take = __builtin_amdgcn_inverse_ballot_w64(actm & okm);
+      const bool open = __builtin_amdgcn_inverse_ballot_w64(om);
+      // wave-uniform: the far-field term is evaluated only when a lane takes it
+      if (actm & okm) P::m2p_sel(acc, r, f, take);
       D = take ? depth : (open ? 0x7fffffff : D);
       if (COUNT) {
         nt += active ? 1 : 0;
         na += take ? 1 : 0;
       }
-      const uint64_t om = __builtin_amdgcn_ballot_w64(active) & ~__builtin_amdgcn_ballot_w64(ok);
       h = om ? 2 * h : r.skip;
     }
   } while (h != 0);
@@ -360,7 +380,8 @@ __global__ __launch_bounds__(NT) void k_walk_forest(const typename P::Rec *__res
                                                     WalkArgs<P> a, const SysDesc *__restrict__ sys,
                                                     const uint2 *__restrict__ tasks,
                                                     uint32_t ntasks, uint32_t *queue) {
-  uint32_t t = (blockIdx.x * NT + threadIdx.x) >> 6;
+  // the wavefront index, uniform within a wavefront (NT is a multiple of 64)
+  uint32_t t = uni((blockIdx.x * NT + threadIdx.x) >> 6);
   for (;;) {
     if (PERSIST) {
       uint32_t got = 0;

@@ -164,6 +164,26 @@ def test_coordinates_sharing_sort_keys(orc, cluster):
     compare_trees(gpu_tree(b), orc.build_tree(b), pscale(b))
 
 
+def test_many_short_key_runs(orc):
+    # thousands of runs of 2 and 3 shared keys at random places in the
+    # sorted lists, so runs start and end on every alignment of the
+    # fix-up pass; half of the pairs have equal coordinates (stable order)
+    rng = np.random.default_rng(11)
+    n = 30000
+    x = rng.uniform(-2.0, 2.0, n)
+    y = rng.uniform(-2.0, 2.0, n)
+    z = rng.normal(0.0, 1e-3, n)
+    idx = rng.permutation(n)
+    a, b2, c = idx[:6000], idx[6000:12000], idx[12000:14000]
+    for arr in (x, y):
+        arr[b2[:3000]] = arr[a[:3000]] + 2.0 ** -48      # distinct, same key
+        arr[b2[3000:]] = arr[a[3000:]]                   # equal coordinates
+        arr[c] = arr[a[:2000]] - 2.0 ** -49              # runs of three
+    b = dict(x=x, y=y, z=z, vx=np.zeros(n), vy=np.zeros(n), vz=np.zeros(n),
+             m=np.full(n, 1.0 / n))
+    compare_trees(gpu_tree(b), orc.build_tree(b), pscale(b))
+
+
 def test_errors():
     b = K.ring(100, seed=0)
     with K.Simulation(50) as s:
