@@ -320,7 +320,7 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
         leaf_batch<P, P::CHUNK, COUNT, PRED>(acc, p, bodies + (r.start + b),
                                              min((uint32_t)P::CHUNK, r.count - b), r.start + b,
                                              me, active, np);
-      if (active) P::fold(acc);
+      P::fold(acc);  // on every lane (a fold is valid at any point): no exec-mask change
       h = r.skip;
     } else {
       // the test runs on every lane (no divergence); only active lanes use it
