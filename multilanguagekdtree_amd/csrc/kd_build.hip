@@ -895,9 +895,18 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
           double hi = a.P.c[d][ids[lst[cur][d][sgs + sgc - 1]]];
           ext[d] = hi - lo;
         }
+        // argmax with ties to the lower dimension, by selects (no indexed
+        // private array)
         int sd = 0;
-        if (ext[1] > ext[sd]) sd = 1;
-        if (ext[2] > ext[sd]) sd = 2;
+        double best = ext[0];
+        if (ext[1] > best) {
+          sd = 1;
+          best = ext[1];
+        }
+        if (ext[2] > best) {
+          sd = 2;
+          best = ext[2];
+        }
         uint32_t mid = sgs + sgc / 2;
         uint16_t *ls = lst[cur][sd];
         const double *key = a.P.c[sd];
@@ -920,7 +929,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
         }
         t.split_dim[h] = (uint8_t)sd;
         t.split_val[h] = kv;
-        t.size[h] = ext[sd];
+        t.size[h] = best;
         sdarr[sgs] = (uint8_t)sd;
       }
     }
