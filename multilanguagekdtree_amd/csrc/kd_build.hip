@@ -191,12 +191,32 @@ __global__ void k_range_init(unsigned long long *range) {
   }
 }
 
+// One block reduces its share to one minimum and one maximum per dimension
+// (wavefront shuffles, then LDS over its four wavefronts); only thread d of
+// each block updates the global range, so the atomics on the six words number
+// 6 per block rather than 6 per wavefront.
+constexpr uint32_t RANGE_BLOCKS = 1024;
+
 __global__ __launch_bounds__(256) void k_range(const double *__restrict__ x,
                                                const double *__restrict__ y,
                                                const double *__restrict__ z, uint32_t n,
                                                unsigned long long *range) {
+  __shared__ uint64_t sh[6][4];
   uint64_t lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  // two positions per iteration, so more loads are in flight
+  for (; i + stride < n; i += 2 * stride) {
+    const double p[6] = {x[i], y[i], z[i], x[i + stride], y[i + stride], z[i + stride]};
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+      const uint64_t k = ordkey(p[e]);
+      const int d = e % 3;
+      lo[d] = k < lo[d] ? k : lo[d];
+      hi[d] = k > hi[d] ? k : hi[d];
+    }
+  }
+  if (i < n) {
     const uint64_t k[3] = {ordkey(x[i]), ordkey(y[i]), ordkey(z[i])};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -213,12 +233,26 @@ __global__ __launch_bounds__(256) void k_range(const double *__restrict__ x,
       hi[d] = b > hi[d] ? b : hi[d];
     }
   }
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      atomicMin(&range[2 * d], (unsigned long long)lo[d]);
-      atomicMax(&range[2 * d + 1], (unsigned long long)hi[d]);
+      sh[2 * d][w] = lo[d];
+      sh[2 * d + 1][w] = hi[d];
     }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int v = threadIdx.x;
+    uint64_t r = sh[v][0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+      const uint64_t c = sh[v][q];
+      r = (v & 1) ? (c > r ? c : r) : (c < r ? c : r);
+    }
+    if (v & 1)
+      atomicMax(&range[v], (unsigned long long)r);
+    else
+      atomicMin(&range[v], (unsigned long long)r);
   }
 }
 
@@ -1239,8 +1273,8 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   const uint32_t nsort = f.nsys == 1 ? f.max_n : f.total;
   if (sort_temp_bytes(nsort, f.nsys) > b.sort_tmp_bytes) return hipErrorInvalidValue;
   k_range_init<<<1, 64, 0, s>>>(b.range);
-  k_range<<<std::min<uint32_t>((nsort + 255) / 256, 2048), 256, 0, s>>>(in.x, in.y, in.z, nsort,
-                                                                         b.range);
+  k_range<<<std::min<uint32_t>((nsort + 255) / 256, RANGE_BLOCKS), 256, 0, s>>>(
+      in.x, in.y, in.z, nsort, b.range);
   for (int d = 0; d < 3; ++d) {
     uint32_t *key32 = reinterpret_cast<uint32_t *>(b.keys_in);
     uint32_t *key32_out = reinterpret_cast<uint32_t *>(b.keys_out);
