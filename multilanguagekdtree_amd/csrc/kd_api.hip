@@ -249,7 +249,7 @@ static int create_impl(kd_ctx *c, int64_t cap, const Sizing &z) {
   ALLOC(c->allocs, c->b.keys_in, N);
   ALLOC(c->allocs, c->b.keys_out, N);
   ALLOC(c->allocs, c->b.ids_in, N);
-  ALLOC(c->allocs, c->b.runs, 2 * (N / 64 + 1) + 2);
+  ALLOC(c->allocs, c->b.runs, 2 * (N / FIX_SHORT + 1) + 2);
   ALLOC(c->allocs, c->b.range, 8);
   c->b.flag_words = (uint32_t)((N + 31) / 32);
   ALLOC(c->allocs, c->b.flags, 32 * (size_t)c->b.flag_words);

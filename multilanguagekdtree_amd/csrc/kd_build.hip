@@ -270,9 +270,8 @@ __global__ void k_make_keys32(const double *__restrict__ p, const unsigned long 
 }
 
 // Runs of equal keys: a run of at most FIX_SHORT ids is put in (coordinate,
-// id) order by its first thread; longer runs whose coordinates are not all
-// equal are queued for k_fix_long (at most n/FIX_SHORT of them).
-constexpr uint32_t FIX_SHORT = 64;
+// id) order by its first thread; the start of a longer run is queued for
+// k_fix_long, which finds its length and checks its coordinates in parallel.
 
 __device__ __forceinline__ bool coord_lt(const double *p, uint32_t a, uint32_t b) {
   const double pa = p[a], pb = p[b];
@@ -284,30 +283,28 @@ __device__ __forceinline__ bool coord_lt(const double *p, uint32_t a, uint32_t b
 __device__ void fix_run(const uint32_t *__restrict__ keys, uint32_t *__restrict__ ids,
                         const double *__restrict__ p, uint32_t n, uint32_t *runs, uint32_t i) {
   const uint32_t k = keys[i];
+  const uint32_t lim = min(n, i + FIX_SHORT + 1);
   uint32_t j = i + 1;
-  bool distinct = false;
-  const double p0 = p[ids[i]];
-  while (j < n && keys[j] == k) {
-    distinct |= p[ids[j]] != p0;
-    ++j;
-  }
-  if (!distinct) return;  // equal coordinates: the stable order stands
-  const uint32_t len = j - i;
-  if (len <= FIX_SHORT) {
-    for (uint32_t a = i + 1; a < j; ++a) {
-      const uint32_t v = ids[a];
-      uint32_t b = a;
-      while (b > i && coord_lt(p, v, ids[b - 1])) {
-        ids[b] = ids[b - 1];
-        --b;
-      }
-      ids[b] = v;
-    }
+  while (j < lim && keys[j] == k) ++j;
+  if (j == i + FIX_SHORT + 1) {  // longer than FIX_SHORT: one workgroup takes it
+    const uint32_t slot = atomicAdd(&runs[0], 1u);
+    runs[1 + 2 * slot] = i;
+    runs[2 + 2 * slot] = 0;
     return;
   }
-  const uint32_t slot = atomicAdd(&runs[0], 1u);
-  runs[1 + 2 * slot] = i;
-  runs[2 + 2 * slot] = len;
+  bool distinct = false;
+  const double p0 = p[ids[i]];
+  for (uint32_t a = i + 1; a < j; ++a) distinct |= p[ids[a]] != p0;
+  if (!distinct) return;  // equal coordinates: the stable order stands
+  for (uint32_t a = i + 1; a < j; ++a) {
+    const uint32_t v = ids[a];
+    uint32_t b = a;
+    while (b > i && coord_lt(p, v, ids[b - 1])) {
+      ids[b] = ids[b - 1];
+      --b;
+    }
+    ids[b] = v;
+  }
 }
 
 // Each thread scans FIX_PER consecutive sorted keys (two 16-byte loads plus
@@ -398,14 +395,36 @@ __device__ void sort_chunk(uint32_t *ids, uint32_t len, const double *__restrict
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) void k_fix_long(const uint32_t *runs, uint32_t *ids,
-                                                  uint32_t *scratch,
+__global__ __launch_bounds__(256) void k_fix_long(const uint32_t *runs,
+                                                  const uint32_t *__restrict__ keys, uint32_t n,
+                                                  uint32_t *ids, uint32_t *scratch,
                                                   const double *__restrict__ p) {
   __shared__ double sk[LONG_LDS];
   __shared__ uint32_t si[LONG_LDS];
+  __shared__ uint32_t s_end;
   const uint32_t nruns = runs[0];
   for (uint32_t r = blockIdx.x; r < nruns; r += gridDim.x) {
-    const uint32_t st = runs[1 + 2 * r], len = runs[2 + 2 * r];
+    const uint32_t st = runs[1 + 2 * r];
+    // the run's end: the first position after st whose key differs
+    const uint32_t k = keys[st];
+    if (threadIdx.x == 0) s_end = n;
+    __syncthreads();
+    uint32_t end = n;
+    for (uint32_t base = st + 1; base < end; base += blockDim.x) {
+      const uint32_t j = base + threadIdx.x;
+      if (j < n && keys[j] != k) atomicMin(&s_end, j);
+      __syncthreads();
+      end = s_end;
+      __syncthreads();
+      if (end < base + blockDim.x) break;
+    }
+    const uint32_t len = end - st;
+    // equal coordinates throughout: the stable order stands
+    const double p0 = p[ids[st]];
+    int distinct = 0;
+    for (uint32_t j = st + 1 + threadIdx.x; j < end; j += blockDim.x)
+      distinct |= p[ids[j]] != p0 ? 1 : 0;
+    if (!__syncthreads_or(distinct)) continue;
     uint32_t *a = ids + st, *b = scratch + st;
     for (uint32_t c = 0; c < len; c += LONG_LDS)
       sort_chunk(a + c, min(LONG_LDS, len - c), p, sk, si);
@@ -1287,7 +1306,7 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
     if (e != hipSuccess) return e;
     k_fix_runs<<<(nsort + 256 * FIX_PER - 1) / (256 * FIX_PER), 256, 0, s>>>(
         key32_out, by_key, coord[d], nsort, b.runs);
-    k_fix_long<<<256, 256, 0, s>>>(b.runs, by_key, b.ids_in, coord[d]);
+    k_fix_long<<<256, 256, 0, s>>>(b.runs, key32_out, nsort, by_key, b.ids_in, coord[d]);
     if (f.nsys > 1) {
       uint32_t *skey = reinterpret_cast<uint32_t *>(b.keys_in);
       uint32_t *skey_out = reinterpret_cast<uint32_t *>(b.keys_out);

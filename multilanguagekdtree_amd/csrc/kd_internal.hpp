@@ -19,6 +19,10 @@ constexpr int LVL_THREADS = 256;
 constexpr int LVL_ITEMS = 8;
 constexpr int LVL_TILE = LVL_THREADS * LVL_ITEMS;  // 2048
 
+// Runs of equal sort keys up to this length are ordered by one thread; longer
+// runs are queued (at most n / (FIX_SHORT + 1) of them) for a workgroup each.
+constexpr uint32_t FIX_SHORT = 16;
+
 constexpr int MAX_LEAF = 32;
 // Tree-ordered body arrays carry this many padding entries so that batched
 // leaf loads (up to 8 bodies) never leave the allocation.

@@ -144,12 +144,12 @@ def test_lattice_ties(orc, shape):
     compare_trees(gpu_tree(b), orc.build_tree(b), pscale(b))
 
 
-@pytest.mark.parametrize("cluster", [40, 3000, 9000])
+@pytest.mark.parametrize("cluster", [16, 17, 40, 3000, 9000])
 def test_coordinates_sharing_sort_keys(orc, cluster):
     # distinct coordinates closer than 2^-32 of the coordinate range share
     # their 32-bit sort key; runs of such keys are ordered by the exact
-    # coordinate (short runs in place, up to 4096 in LDS, longer ones by a
-    # merge sort)
+    # coordinate (up to 16 by one thread, up to 4096 in LDS, longer ones by
+    # a merge sort)
     rng = np.random.default_rng(cluster)
     n = 20000
     x = rng.uniform(-2.0, 2.0, n)
