@@ -170,7 +170,7 @@ __global__ void k_shape(uint32_t *seg_start, uint32_t *seg_count, uint8_t *lmask
 // ---------------------------------------------------------------------------
 
 // The id lists are sorted by a 32-bit fixed-point image of each coordinate
-// over its current range, key = floor((x - min) (2^32 - 1) / (max - min))
+// over its current range, key = floor((x - min) (2^32 - 2) / (max - min))
 // (computed on halves, so no range overflows).  Every rounding step of the
 // map is monotone, so the key order never contradicts the coordinate order;
 // bodies whose distinct coordinates share a key are put in coordinate order
@@ -256,16 +256,28 @@ __global__ __launch_bounds__(256) void k_range(const double *__restrict__ x,
   }
 }
 
+// Body keys span [0, PAD_KEY - 1]; in a forest, the padding positions that
+// round each system up to whole wavefronts take PAD_KEY, so they sort last
+// and never share a key with a body (their order is irrelevant: the system
+// sort moves them behind every system's bodies).
+constexpr uint32_t PAD_KEY = 0xFFFFFFFFu;
+
 __global__ void k_make_keys32(const double *__restrict__ p, const unsigned long long *range,
                               uint32_t *__restrict__ keys, uint32_t *__restrict__ ids, uint32_t n,
-                              uint32_t *runs) {
+                              uint32_t *runs, const uint32_t *__restrict__ sob,
+                              const SysDesc *__restrict__ sys) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j == 0) runs[0] = 0;
   if (j >= n) return;
   const double lo = 0.5 * ordkey_inv(range[0]), hi = 0.5 * ordkey_inv(range[1]);
-  const double scale = hi > lo ? 4294967295.0 / (hi - lo) : 0.0;
+  const double scale = hi > lo ? 4294967294.0 / (hi - lo) : 0.0;
   const double v = 0.5 * p[j];
-  keys[j] = (uint32_t)fmin(fmax((v - lo) * scale, 0.0), 4294967295.0);
+  uint32_t k = (uint32_t)fmin(fmax((v - lo) * scale, 0.0), 4294967294.0);
+  if (sob != nullptr) {
+    const SysDesc S = sys[sob[j >> 6]];
+    if (j - S.body_off >= S.n) k = PAD_KEY;
+  }
+  keys[j] = k;
   ids[j] = j;
 }
 
@@ -283,6 +295,7 @@ __device__ __forceinline__ bool coord_lt(const double *p, uint32_t a, uint32_t b
 __device__ void fix_run(const uint32_t *__restrict__ keys, uint32_t *__restrict__ ids,
                         const double *__restrict__ p, uint32_t n, uint32_t *runs, uint32_t i) {
   const uint32_t k = keys[i];
+  if (k == PAD_KEY) return;  // forest padding only
   const uint32_t lim = min(n, i + FIX_SHORT + 1);
   uint32_t j = i + 1;
   while (j < lim && keys[j] == k) ++j;
@@ -407,6 +420,7 @@ __global__ __launch_bounds__(256) void k_fix_long(const uint32_t *runs,
     const uint32_t st = runs[1 + 2 * r];
     // the run's end: the first position after st whose key differs
     const uint32_t k = keys[st];
+    if (k == PAD_KEY) continue;  // forest padding only (uniform branch)
     if (threadIdx.x == 0) s_end = n;
     __syncthreads();
     uint32_t end = n;
@@ -1297,8 +1311,9 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   for (int d = 0; d < 3; ++d) {
     uint32_t *key32 = reinterpret_cast<uint32_t *>(b.keys_in);
     uint32_t *key32_out = reinterpret_cast<uint32_t *>(b.keys_out);
-    k_make_keys32<<<(nsort + 255) / 256, 256, 0, s>>>(coord[d], b.range + 2 * d, key32, b.ids_in,
-                                                      nsort, b.runs);
+    k_make_keys32<<<(nsort + 255) / 256, 256, 0, s>>>(
+        coord[d], b.range + 2 * d, key32, b.ids_in, nsort, b.runs,
+        f.nsys > 1 ? f.sys_of_block : nullptr, f.sys);
     size_t bytes = b.sort_tmp_bytes;
     uint32_t *by_key = f.nsys == 1 ? b.L[d] : b.Lb[d];
     e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, key32, key32_out, b.ids_in, by_key, nsort,
