@@ -1132,26 +1132,34 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   }
   __threadfence_block();
   __syncthreads();
-  // leaf moments, summed in canonical order from the tree-ordered state
+  // Leaf moments, summed in canonical order (tree positions st .. st+ct-1)
+  // from the tree-ordered state: one quantity per pass (m, m x, m y, m z) is
+  // staged in LDS by all threads (the list buffers are free now), then each
+  // leaf's first thread adds its members' entries in order.
+  double *stg = reinterpret_cast<double *>(&lst[0][0][0]);
+  static_assert(sizeof(lst) >= SUB_MAX * sizeof(double), "staging needs the list buffers");
+  for (int q = 0; q < 4; ++q) {
+    const double *pc = q == 1 ? a.out.x : (q == 2 ? a.out.y : a.out.z);
+    for (uint32_t i = tid; i < n0; i += NT) {
+      const double bm = a.out.m[s0 + i];
+      stg[i] = q == 0 ? bm : bm * pc[s0 + i];
+    }
+    __syncthreads();
+    double *dst = q == 0 ? t.mm : (q == 1 ? t.mx : (q == 2 ? t.my : t.mz));
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const uint32_t j = tid * IT + it;
+      if (j >= n0 || j != st[it]) continue;
+      double sum = 0.0;
+      for (uint32_t r = 0; r < ct[it]; ++r) sum += stg[st[it] + r];
+      dst[hl[it]] = sum;
+    }
+    __syncthreads();
+  }
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
-    uint32_t j = tid * IT + it;
-    if (j >= n0 || j != st[it]) continue;
-    double mm = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
-    for (uint32_t q = 0; q < ct[it]; ++q) {
-      const uint32_t tp = s0 + st[it] + q;
-      const double bm = a.out.m[tp];
-      mm += bm;
-      mx += bm * a.out.x[tp];
-      my += bm * a.out.y[tp];
-      mz += bm * a.out.z[tp];
-    }
-    const uint32_t h = hl[it];
-    t.mm[h] = mm;
-    t.mx[h] = mx;
-    t.my[h] = my;
-    t.mz[h] = mz;
-    write_leaf_rec(t, h, s0l + st[it], ct[it]);
+    const uint32_t j = tid * IT + it;
+    if (j < n0 && j == st[it]) write_leaf_rec(t, hl[it], s0l + st[it], ct[it]);
   }
 
   // moments bottom-up over this subtree's internal nodes
