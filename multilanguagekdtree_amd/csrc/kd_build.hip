@@ -328,9 +328,13 @@ constexpr uint32_t FIX_PER = 8;
 __global__ __launch_bounds__(256) void k_fix_runs(const uint32_t *__restrict__ keys,
                                                   uint32_t *__restrict__ ids,
                                                   const double *__restrict__ p, uint32_t n,
+                                                  const unsigned long long *range,
                                                   uint32_t *runs) {
   const uint32_t i0 = (blockIdx.x * blockDim.x + threadIdx.x) * FIX_PER;
   if (i0 >= n) return;
+  // every coordinate of this dimension equal (a planar system's z): the
+  // stable order of the sort is already the exact one
+  if (range[0] == range[1]) return;
   uint32_t k[FIX_PER + 2];  // k[0] = keys[i0 - 1], k[1 + j] = keys[i0 + j]
   const uint32_t m = min(FIX_PER, n - i0);
   if (m == FIX_PER) {
@@ -1328,7 +1332,7 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
                                   0, 32, s);
     if (e != hipSuccess) return e;
     k_fix_runs<<<(nsort + 256 * FIX_PER - 1) / (256 * FIX_PER), 256, 0, s>>>(
-        key32_out, by_key, coord[d], nsort, b.runs);
+        key32_out, by_key, coord[d], nsort, b.range + 2 * d, b.runs);
     k_fix_long<<<256, 256, 0, s>>>(b.runs, key32_out, nsort, by_key, b.ids_in, coord[d]);
     if (f.nsys > 1) {
       uint32_t *skey = reinterpret_cast<uint32_t *>(b.keys_in);
