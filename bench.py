@@ -427,10 +427,24 @@ def main():
             traffic = None if t is None else float(t["bytes"])
         except Exception:
             traffic = None
+    # vector-ALU counters of the walk from the committed counter pass
+    # (profiles/valu_ring10M_mixed.md): VALUBusy and VALUUtilization
+    valu = None
+    vf = ROOT / "profiles" / "valu.json"
+    if vf.exists() and main_name == "mixed":
+        try:
+            valu = json.loads(vf.read_text()).get(f"traverse_{main_name}_n{n}")
+        except Exception:
+            valu = None
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
                     frac=round(achieved / peak, 4), traffic=traffic,
                     kernel=f"k_traverse_{'mixed' if main_name == 'mixed' else 'f64'}",
                     kernel_ms=round(r["traverse_ms"], 3),
+                    hbm_gbps=(None if traffic is None
+                              else round(traffic / (r["traverse_ms"] * 1e-3) / 1e9, 1)),
+                    valu_busy_pct=None if valu is None else round(valu["valu_busy_pct"], 1),
+                    valu_utilization_pct=(None if valu is None
+                                          else round(valu["valu_utilization_pct"], 1)),
                     flops_per_launch=flops,
                     per_particle=dict(tests=r["tests"] / n, m2p=r["m2p"] / n, p2p=r["p2p"] / n))
 
