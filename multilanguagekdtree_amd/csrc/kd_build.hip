@@ -3,12 +3,15 @@
 // The tree shape is a pure function of N (every split is at floor(len/2),
 // Sequential/RustVersion/src/array_kd_tree.rs:100), so segment bounds are
 // known up front and the tree is kept in heap order.  Each step:
-//   1. three lists of body ids sorted by (x), (y), (z) (device radix sort);
+//   1. three lists of body ids sorted by (x), (y), (z): a device radix sort
+//      on 32-bit fixed-point keys over each coordinate's range, then runs of
+//      shared keys put in exact (coordinate, id) order;
 //   2. device-wide levels while segments exceed SUB_MAX bodies: per segment,
 //      bbox from the ends of the three sorted lists, split dimension with the
 //      reference's strict '>' (:91-96), split value = key at mid (:125); the
 //      split-dim list is already partitioned at mid, the other two are
-//      stably partitioned by a left/right flag (segmented scan + scatter);
+//      stably partitioned by a left/right bit per id (per-tile counts, an
+//      exclusive scan over tiles, scatter);
 //   3. one workgroup per remaining segment finishes its subtree in LDS,
 //      emits leaves in canonical order (ascending original index), node
 //      moments bottom-up, and writes the body state in tree order;

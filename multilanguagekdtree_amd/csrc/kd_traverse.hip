@@ -17,8 +17,10 @@
 //   F32   FP32 coordinates, v_rsq_f32, opening threshold size^2/theta^2
 //         baked into the record, blocked compensated (Kahan) FP32 sums:
 //         terms accumulate in a plain FP32 partial that is folded into the
-//         (sum, compensation) pair after every leaf;
-//   D64   as F32 but the coordinate difference is taken in FP64.
+//         (sum, compensation) pair after every leaf; the self pair adds 0
+//         through a per-body floor on d2 instead of an index compare;
+//   D64   as F32 but the coordinate difference is taken in FP64 (self pair
+//         dropped by index, as F64).
 #include <algorithm>
 
 #include "kd_internal.hpp"
