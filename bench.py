@@ -346,7 +346,7 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-every", type=int, default=20)
     ap.add_argument("--adj-dirs", type=int, default=64)
-    ap.add_argument("--adj-dirs-fp64", type=int, default=8)
+    ap.add_argument("--adj-dirs-fp64", type=int, default=64)
     ap.add_argument("--adj-cpu-n", type=int, default=2000)
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=20)
