@@ -248,10 +248,8 @@ struct WalkArgs {
 };
 
 // P2P of the lane's body with n (1..B) consecutive leaf bodies starting at
-// tree position s0, entered through a switch so the scalar unit spends one
-// computed branch per batch instead of a compare-and-branch per body.  The
-// batch loads are unconditional (the body arrays carry BODY_PAD spare
-// entries); terms are added from the last body to the first.
+// tree position s0.  The batch loads are unconditional (the body arrays
+// carry BODY_PAD spare entries); terms are added in leaf order.
 template <class P, int B, bool COUNT, bool PRED>
 __device__ __forceinline__ void leaf_batch(typename P::Acc &acc, const typename P::Self &p,
                                            const typename P::Body *__restrict__ base,
@@ -271,25 +269,15 @@ __device__ __forceinline__ void leaf_batch(typename P::Acc &acc, const typename 
     }
     return;
   }
-#define KD_P2P(j)                                          \
-  case (j) + 1:                                            \
-    P::p2p(acc, p, q[(j) % B], P::SELF_BY_INDEX && s0 + (j) == me); \
-    if (COUNT) np += (s0 + (j) != me) ? 1 : 0;             \
-    [[fallthrough]];
-  switch (n) {
-    default:
-    if (B >= 8) { KD_P2P(7) }
-    if (B >= 7) { KD_P2P(6) }
-    if (B >= 6) { KD_P2P(5) }
-    if (B >= 5) { KD_P2P(4) }
-    KD_P2P(3)
-    KD_P2P(2)
-    KD_P2P(1)
-    KD_P2P(0)
-    case 0:
-      break;
+  // one wave-uniform branch per body, terms in leaf order; each term updates
+  // the accumulators in place, so the joins need no register copies
+#pragma unroll
+  for (int j = 0; j < B; ++j) {
+    if ((uint32_t)j < n) {
+      P::p2p(acc, p, q[j], P::SELF_BY_INDEX && s0 + j == me);
+      if (COUNT) np += (s0 + j != me) ? 1 : 0;
+    }
   }
-#undef KD_P2P
 }
 
 // The walk of one wavefront: bodies [wbase, wbase + 64) of a tree whose
