@@ -201,6 +201,33 @@ def test_errors():
         with pytest.raises(_lib.KdError) as e:
             s.set_params(leaf_size=1)
         assert e.value.code == _lib.KD_ERR_ARG
+        inf = K.ring(50, seed=0)
+        inf["vz"][3] = np.inf
+        with pytest.raises(_lib.KdError) as e:
+            s.upload(inf)
+        assert e.value.code == _lib.KD_ERR_NONFINITE
+        with pytest.raises(_lib.KdError) as e:
+            s.upload({k: v[:0] for k, v in b.items()})
+        assert e.value.code == _lib.KD_ERR_ARG
+
+
+def test_coincident_bodies():
+    # two bodies at one position: the reference's pair term divides by zero
+    # (array_particle.rs:71-73); the FP64 step reports the non-finite state
+    # instead of failing later, the mixed step (per-body floor) carries on
+    n = 1000
+    b = {k: v.copy() for k, v in K.ring(n, seed=1).items()}
+    for f in ("x", "y", "z"):
+        b[f][5] = b[f][600]
+    with K.Simulation(n) as s:
+        s.upload(b)
+        with pytest.raises(_lib.KdError) as e:
+            s.step(1)
+        assert e.value.code == _lib.KD_ERR_NONFINITE
+    with K.Simulation(n, precision=K.KD_PREC_MIXED) as s:
+        s.upload(b)
+        s.step(3)
+        assert all(np.isfinite(v).all() for v in s.download().values())
 
 
 # ---- golden vectors of the reference's own C build (tests/golden) ----------
