@@ -16,6 +16,8 @@
 // dc/dm_j = (p_j - c) / M, dM/dm_j = 1.  All adjoint sums are FP64 (global
 // FP64 atomics); the mixed path evaluates its terms in FP32 like its forward
 // walk.  Adjoint arrays are in original body order.
+#include <rocprim/warp/warp_reduce.hpp>
+
 #include <algorithm>
 
 #include "kd_internal.hpp"
@@ -26,10 +28,15 @@ __device__ __forceinline__ uint32_t uni_a(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
 }
 
+// Sum over the 64 lanes of a wavefront, on every lane: rocprim's DPP
+// reduction (row rotations and broadcasts move the two 32-bit halves in
+// registers; one final cross-lane read), instead of six LDS-routed shuffles.
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  using WR = rocprim::warp_reduce<double, 64>;
+  WR::storage_type st;
+  double out;
+  WR().reduce(v, out, st);
+  return out;
 }
 
 __device__ __forceinline__ uint32_t heap_next_a(uint32_t h, int &depth) {
