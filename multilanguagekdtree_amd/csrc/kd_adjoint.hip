@@ -113,7 +113,7 @@ template <bool MIXED>
 __global__ __launch_bounds__(256) void k_adj_walk(AdjWalk a) {
 #pragma clang fp contract(off)
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+  const uint32_t wbase = (xcd_block() * blockDim.x + threadIdx.x) & ~63u;
   if (wbase >= a.n) return;
   const uint32_t me = wbase + lane;
   const bool valid = me < a.n;

@@ -24,6 +24,17 @@ constexpr int LVL_TILE = LVL_THREADS * LVL_ITEMS;  // 2048
 constexpr uint32_t FIX_SHORT = 16;
 
 constexpr int MAX_LEAF = 32;
+
+// The block index a kernel should treat as its own so that each of the 8
+// XCDs runs one contiguous range of the grid: the dispatcher hands blocks to
+// the XCDs round robin (block b to XCD b mod 8), and neighbouring bodies
+// share tree nodes and leaf bodies that each XCD caches in its own L2.  A
+// bijection of [0, gridDim.x).
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, q = nb / 8u, r = nb % 8u;
+  const uint32_t x = blockIdx.x % 8u, i = blockIdx.x / 8u;
+  return x * q + (x < r ? x : r) + i;
+}
 // Tree-ordered body arrays carry this many padding entries so that batched
 // leaf loads (up to 8 bodies) never leave the allocation.
 constexpr int BODY_PAD = 32;

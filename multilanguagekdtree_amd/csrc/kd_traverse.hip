@@ -355,7 +355,7 @@ template <class P, bool COUNT, bool PRED>
 __global__ __launch_bounds__(256) void k_walk(const typename P::Rec *__restrict__ nodes_m1,
                                               const typename P::Body *__restrict__ bodies,
                                               WalkArgs<P> a) {
-  const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+  const uint32_t wbase = (xcd_block() * blockDim.x + threadIdx.x) & ~63u;
   if (wbase >= a.n) return;
   walk_wave<P, COUNT, PRED>(nodes_m1, bodies, a, wbase);
 }
@@ -371,7 +371,7 @@ __global__ __launch_bounds__(NT) void k_walk_forest(const typename P::Rec *__res
                                                     const uint2 *__restrict__ tasks,
                                                     uint32_t ntasks, uint32_t *queue) {
   // the wavefront index, uniform within a wavefront (NT is a multiple of 64)
-  uint32_t t = uni((blockIdx.x * NT + threadIdx.x) >> 6);
+  uint32_t t = uni(((PERSIST ? blockIdx.x : xcd_block()) * NT + threadIdx.x) >> 6);
   for (;;) {
     if (PERSIST) {
       uint32_t got = 0;
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(256) void k_walk_f64(const NodeRec64 *__restrict__ 
                                                   WalkArgs<F64> a) {
 #pragma clang fp contract(off)
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t wbase = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+  const uint32_t wbase = (xcd_block() * blockDim.x + threadIdx.x) & ~63u;
   if (wbase >= a.n) return;
   const uint32_t me = wbase + lane;
   const bool valid = me < a.n;
