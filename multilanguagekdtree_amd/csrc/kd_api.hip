@@ -206,6 +206,11 @@ int kd_destroy(kd_ctx *c) {
   for (void *p : c->allocs) (void)hipFree(p);
   for (void *p : c->node_allocs) (void)hipFree(p);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
+  for (int i = 0; i < 2; ++i) {
+    if (c->b.side[i]) (void)hipStreamDestroy(c->b.side[i]);
+    if (c->b.join[i]) (void)hipEventDestroy(c->b.join[i]);
+  }
+  if (c->b.fork) (void)hipEventDestroy(c->b.fork);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return KD_OK;
@@ -246,10 +251,19 @@ static int create_impl(kd_ctx *c, int64_t cap, const Sizing &z) {
     ALLOC(c->allocs, c->acc[d], N);
     ALLOC(c->allocs, c->acc_o[d], N);
   }
-  ALLOC(c->allocs, c->b.keys_in, N);
-  ALLOC(c->allocs, c->b.keys_out, N);
-  ALLOC(c->allocs, c->b.ids_in, N);
-  ALLOC(c->allocs, c->b.runs, 2 * (N / FIX_SHORT + 1) + 2);
+  c->b.sort_tmp_bytes = z.sort_bytes;
+  for (int d = 0; d < 3; ++d) {
+    ALLOC(c->allocs, c->b.keys_in[d], N);
+    ALLOC(c->allocs, c->b.keys_out[d], N);
+    ALLOC(c->allocs, c->b.ids_in[d], N);
+    ALLOC(c->allocs, c->b.runs[d], 2 * (N / FIX_SHORT + 1) + 2);
+    ALLOC(c->allocs, *(uint8_t **)&c->b.sort_tmp[d], c->b.sort_tmp_bytes);
+  }
+  for (int i = 0; i < 2; ++i) {
+    HIP_TRY(hipStreamCreateWithFlags(&c->b.side[i], hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c->b.join[i], hipEventDisableTiming));
+  }
+  HIP_TRY(hipEventCreateWithFlags(&c->b.fork, hipEventDisableTiming));
   ALLOC(c->allocs, c->b.range, 8);
   c->b.flag_words = (uint32_t)((N + 31) / 32);
   ALLOC(c->allocs, c->b.flags, 32 * (size_t)c->b.flag_words);
@@ -264,8 +278,6 @@ static int create_impl(kd_ctx *c, int64_t cap, const Sizing &z) {
   HIP_TRY(hipMemsetAsync(c->pos32, 0, sizeof(float4) * (N + BODY_PAD), c->stream));
   ALLOC(c->allocs, c->b.cnt, z.cnt_entries);
   ALLOC(c->allocs, c->b.tieflag, z.tieflag_entries);
-  c->b.sort_tmp_bytes = z.sort_bytes;
-  ALLOC(c->allocs, *(uint8_t **)&c->b.sort_tmp, c->b.sort_tmp_bytes);
   ALLOC(c->allocs, c->dsys, z.nsys);
   ALLOC(c->allocs, c->sob, N / 64 + 1);
   ALLOC(c->allocs, c->counters, 8);

@@ -1266,6 +1266,16 @@ SysDesc make_sys(const Shape &sh, uint32_t body_off, uint32_t node_off, uint32_t
   return d;
 }
 
+// Radix sort of (32-bit key, 32-bit id) pairs: 8-bit digits, 1024 threads x
+// 10 items per tile.  On gfx950 this sorts 10 M pairs in ~320 us, against
+// ~400 us for rocprim's default tile of 1024 x 16 (wider digits, 10 or 11
+// bits in three passes, measured slower).
+using SortCfg = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 10>,
+                                        rocprim::kernel_config<1024, 10>, 8,
+                                        rocprim::block_radix_rank_algorithm::match>>;
+
 static int bits_for(uint32_t v) {
   int b = 1;
   while (b < 32 && (v >> b) != 0) ++b;
@@ -1274,13 +1284,14 @@ static int bits_for(uint32_t v) {
 
 size_t sort_temp_bytes(uint32_t n, int nsys) {
   size_t bytes = 0;
-  (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                  (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0, 32);
+  (void)rocprim::radix_sort_pairs<SortCfg>(nullptr, bytes, (uint32_t *)nullptr,
+                                           (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                           (uint32_t *)nullptr, n, 0, 32);
   if (nsys > 1) {
     size_t b2 = 0;
-    (void)rocprim::radix_sort_pairs(nullptr, b2, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                    (uint32_t *)nullptr, (uint32_t *)nullptr, n, 0,
-                                    bits_for((uint32_t)nsys));
+    (void)rocprim::radix_sort_pairs<SortCfg>(nullptr, b2, (uint32_t *)nullptr,
+                                             (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                             (uint32_t *)nullptr, n, 0, bits_for((uint32_t)nsys));
     bytes = std::max(bytes, b2);
   }
   return bytes;
@@ -1323,31 +1334,43 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   k_range_init<<<1, 64, 0, s>>>(b.range);
   k_range<<<std::min<uint32_t>((nsort + 255) / 256, RANGE_BLOCKS), 256, 0, s>>>(
       in.x, in.y, in.z, nsort, b.range);
+  // the three coordinate sorts are independent: dimension 0 runs on s,
+  // dimensions 1 and 2 on the side streams, forked after the range and
+  // joined before the levels (a graph capture of s follows the fork)
+  e = hipEventRecord(b.fork, s);
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipStreamWaitEvent(b.side[i], b.fork, 0);
+  if (e != hipSuccess) return e;
   for (int d = 0; d < 3; ++d) {
-    uint32_t *key32 = reinterpret_cast<uint32_t *>(b.keys_in);
-    uint32_t *key32_out = reinterpret_cast<uint32_t *>(b.keys_out);
-    k_make_keys32<<<(nsort + 255) / 256, 256, 0, s>>>(
-        coord[d], b.range + 2 * d, key32, b.ids_in, nsort, b.runs,
+    const hipStream_t sd = d == 0 ? s : b.side[d - 1];
+    uint32_t *key32 = b.keys_in[d];
+    uint32_t *key32_out = b.keys_out[d];
+    k_make_keys32<<<(nsort + 255) / 256, 256, 0, sd>>>(
+        coord[d], b.range + 2 * d, key32, b.ids_in[d], nsort, b.runs[d],
         f.nsys > 1 ? f.sys_of_block : nullptr, f.sys);
     size_t bytes = b.sort_tmp_bytes;
     uint32_t *by_key = f.nsys == 1 ? b.L[d] : b.Lb[d];
-    e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, key32, key32_out, b.ids_in, by_key, nsort,
-                                  0, 32, s);
+    e = rocprim::radix_sort_pairs<SortCfg>(b.sort_tmp[d], bytes, key32, key32_out, b.ids_in[d],
+                                           by_key, nsort, 0, 32, sd);
     if (e != hipSuccess) return e;
-    k_fix_runs<<<(nsort + 256 * FIX_PER - 1) / (256 * FIX_PER), 256, 0, s>>>(
-        key32_out, by_key, coord[d], nsort, b.range + 2 * d, b.runs);
-    k_fix_long<<<256, 256, 0, s>>>(b.runs, key32_out, nsort, by_key, b.ids_in, coord[d]);
+    k_fix_runs<<<(nsort + 256 * FIX_PER - 1) / (256 * FIX_PER), 256, 0, sd>>>(
+        key32_out, by_key, coord[d], nsort, b.range + 2 * d, b.runs[d]);
+    k_fix_long<<<256, 256, 0, sd>>>(b.runs[d], key32_out, nsort, by_key, b.ids_in[d], coord[d]);
     if (f.nsys > 1) {
-      uint32_t *skey = reinterpret_cast<uint32_t *>(b.keys_in);
-      uint32_t *skey_out = reinterpret_cast<uint32_t *>(b.keys_out);
-      k_sys_keys<<<(nsort + 255) / 256, 256, 0, s>>>(by_key, f.sys_of_block, f.sys,
-                                                     (uint32_t)f.nsys, skey, nsort);
+      uint32_t *skey = b.keys_in[d];
+      uint32_t *skey_out = b.keys_out[d];
+      k_sys_keys<<<(nsort + 255) / 256, 256, 0, sd>>>(by_key, f.sys_of_block, f.sys,
+                                                      (uint32_t)f.nsys, skey, nsort);
       bytes = b.sort_tmp_bytes;
-      e = rocprim::radix_sort_pairs(b.sort_tmp, bytes, skey, skey_out, by_key, b.L[d], nsort, 0,
-                                    bits_for((uint32_t)f.nsys), s);
+      e = rocprim::radix_sort_pairs<SortCfg>(b.sort_tmp[d], bytes, skey, skey_out, by_key, b.L[d],
+                                             nsort, 0, bits_for((uint32_t)f.nsys), sd);
       if (e != hipSuccess) return e;
     }
   }
+  for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+    e = hipEventRecord(b.join[i], b.side[i]);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, b.join[i], 0);
+  }
+  if (e != hipSuccess) return e;
   ListBufs LB;
   LB.A = Lists{{b.L[0], b.L[1], b.L[2]}};
   LB.B = Lists{{b.Lb[0], b.Lb[1], b.Lb[2]}};

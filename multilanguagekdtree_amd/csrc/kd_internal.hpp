@@ -152,12 +152,16 @@ struct Shape {
 
 struct BuildBufs {
   uint32_t *L[3], *Lb[3];  // sorted id lists and their partition buffers
-  uint64_t *keys_in, *keys_out;
-  uint32_t *ids_in;
+  // per dimension, so that the three coordinate sorts run concurrently:
+  // dimension 0 on the build stream, 1 and 2 on the side streams
+  uint32_t *keys_in[3], *keys_out[3];
+  uint32_t *ids_in[3];
+  uint32_t *runs[3];  // long runs of equal sort keys: count, then (start, length)
+  void *sort_tmp[3];
+  size_t sort_tmp_bytes;  // of each sort_tmp[d]
+  hipStream_t side[2];    // default priority, like the build stream
+  hipEvent_t fork, join[2];
   unsigned long long *range;  // ordered-key minimum and maximum of x, y, z
-  uint32_t *runs;             // long runs of equal sort keys: count, then (start, length)
-  void *sort_tmp;
-  size_t sort_tmp_bytes;
   uint8_t *flags;    // left/right flag per working id (capacity rounded up to 32)
   uint32_t *bits;    // the flags packed, one bit per working id
   uint32_t flag_words;  // 32-bit words of bits

@@ -4,9 +4,9 @@
 // heap-ordered tree depth first without a stack: the current node h
 // (1-based) is wave-uniform, so node records and leaf bodies arrive through
 // scalar loads; descending goes to 2h, finishing a subtree goes to
-// (h >> trailing_ones(h)) + 1.  Each lane keeps the depth D at which it last
-// accepted a node (INT_MAX while it keeps opening): a lane takes part in node
-// h iff depth(h) <= D, so every lane sees exactly the nodes and leaves its
+// (h >> trailing_ones(h)) + 1.  Each lane keeps the level D (root 1) at which
+// it last accepted a node (~0u while it keeps opening): a lane takes part in
+// node h iff level(h) <= D, so every lane sees exactly the nodes and leaves its
 // own recursive traversal (Sequential/RustVersion/src/array_kd_tree.rs:
 // 140-172) sees.  The wave descends when any active lane opens.
 //
@@ -17,7 +17,8 @@
 //   F32   FP32 coordinates, v_rsq_f32, opening threshold size^2/theta^2
 //         baked into the record, blocked compensated (Kahan) FP32 sums:
 //         terms accumulate in a plain FP32 partial that is folded into the
-//         (sum, compensation) pair after every leaf; the self pair adds 0
+//         compensated sum after every leaf (the partial then carries the
+//         negated compensation into the next block); the self pair adds 0
 //         through a per-body floor on d2 instead of an index compare;
 //   D64   as F32 but the coordinate difference is taken in FP64 (self pair
 //         dropped by index, as F64).
@@ -116,20 +117,21 @@ struct F64 {
   __device__ static double result(const Acc &a, int i) { return a.s[i]; }
 };
 
+// Blocked Kahan summation with the compensation carried in the partial:
+// after a fold the partial holds -c (the rounding error of the last sum,
+// negated), and the next block of terms is added on top of it.  A fold is
+// t = s + p; p -= t - s; s = t: three operations per component and no reset.
 struct KahanAcc {
-  float p[3] = {0.f, 0.f, 0.f};  // partial of the current block of terms
+  float p[3] = {0.f, 0.f, 0.f};  // the current block of terms minus the compensation
   float s[3] = {0.f, 0.f, 0.f};  // compensated sum
-  float c[3] = {0.f, 0.f, 0.f};  // running compensation
 };
 
 __device__ __forceinline__ void kahan_fold(KahanAcc &a) {
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-    float y = a.p[i] - a.c[i];
-    float t = a.s[i] + y;
-    a.c[i] = (t - a.s[i]) - y;
+    const float t = a.s[i] + a.p[i];
+    a.p[i] -= t - a.s[i];
     a.s[i] = t;
-    a.p[i] = 0.f;
   }
 }
 
@@ -184,7 +186,7 @@ struct F32 {
   }
   __device__ static void fold(Acc &a) { kahan_fold(a); }
   __device__ static double result(const Acc &a, int i) {
-    return ((double)a.s[i] + (double)a.p[i]) - (double)a.c[i];
+    return (double)a.s[i] + (double)a.p[i];
   }
 };
 
@@ -230,7 +232,7 @@ struct D64 {
   }
   __device__ static void fold(Acc &a) { kahan_fold(a); }
   __device__ static double result(const Acc &a, int i) {
-    return ((double)a.s[i] + (double)a.p[i]) - (double)a.c[i];
+    return (double)a.s[i] + (double)a.p[i];
   }
 };
 
@@ -293,15 +295,17 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
   const bool valid = me < a.n;
   const typename P::Self p = P::self(bodies[valid ? me : wbase]);
   typename P::Acc acc;
-  int D = valid ? 0x7fffffff : -1;
+  // levels count from 1 at the root, so D = 0 keeps a lane out of every node
+  // and ~0u (while it keeps opening) lets it into every node
+  uint32_t D = valid ? ~0u : 0u;
   unsigned long long nt = 0, na = 0, np = 0, wv = 0, wl = 0;
   uint32_t h = 1;
   do {
     const typename P::Rec r = load_rec(nodes_m1, uni(h));
-    const int depth = 31 - __builtin_clz(h);
+    const uint32_t level = 32 - __builtin_clz(h);
     // the lanes taking part, as a scalar mask (kept on the scalar unit) and
     // as the lane's own condition
-    const uint64_t actm = __builtin_amdgcn_ballot_w64(depth <= D);
+    const uint64_t actm = __builtin_amdgcn_ballot_w64(level <= D);
     const bool active = __builtin_amdgcn_inverse_ballot_w64(actm);
     if (COUNT) ++wv;
     if (r.count <= a.leaf) {
@@ -323,7 +327,8 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
       const bool open = __builtin_amdgcn_inverse_ballot_w64(om);
       // wave-uniform: the far-field term is evaluated only when a lane takes it
       if (actm & okm) P::m2p_sel(acc, r, f, take);
-      D = take ? depth : (open ? 0x7fffffff : D);
+      // a taking lane had D >= level, a lane outside D < level
+      D = open ? ~0u : min(D, level);
       if (COUNT) {
         nt += active ? 1 : 0;
         na += take ? 1 : 0;
