@@ -653,11 +653,18 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_flags(ListBufs LB, TreeDe
   uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
   uint32_t mid = s + c / 2, e = s + c;
+  if (c == 0) return;
   const uint32_t *ls = LB.in(t.lmask[h], sd);
   uint32_t base = s + q * LVL_TILE;
+  // every load first (indices clamped into the segment, so all are in
+  // bounds and independent), then the stores of the items that exist
+  uint32_t id[LVL_ITEMS];
+#pragma unroll
+  for (int i = 0; i < LVL_ITEMS; ++i) id[i] = ls[min(base + i * LVL_THREADS + threadIdx.x, e - 1)];
+#pragma unroll
   for (int i = 0; i < LVL_ITEMS; ++i) {
     uint32_t j = base + i * LVL_THREADS + threadIdx.x;
-    if (j < e) flags[ls[j]] = j < mid ? 1 : 0;
+    if (j < e) flags[id[i]] = j < mid ? 1 : 0;
   }
 }
 
@@ -705,11 +712,20 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_count(ListBufs LB, TreeDe
   uint32_t base = s + q * LVL_TILE;
   for (int d = 0; d < 3; ++d) {
     uint32_t local = 0;
-    if (d != sd) {
+    if (d != sd && c != 0) {
+      // list entries, then their flag words, each as a batch of independent
+      // loads (indices clamped into the segment); items past e count 0
       const uint32_t *ld = LB.in(t.lmask[h], d);
+      uint32_t id[LVL_ITEMS], w[LVL_ITEMS];
+#pragma unroll
+      for (int i = 0; i < LVL_ITEMS; ++i)
+        id[i] = ld[min(base + i * LVL_THREADS + threadIdx.x, e - 1)];
+#pragma unroll
+      for (int i = 0; i < LVL_ITEMS; ++i) w[i] = bits[id[i] >> 5];
+#pragma unroll
       for (int i = 0; i < LVL_ITEMS; ++i) {
         uint32_t j = base + i * LVL_THREADS + threadIdx.x;
-        if (j < e) local += flag_bit(bits, ld[j]);
+        local += j < e ? (w[i] >> (id[i] & 31u)) & 1u : 0u;
       }
     }
     uint32_t tot;
@@ -753,6 +769,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(ListBufs LB,
   uint32_t h = nseg - 1 + k;
   uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
   int sd = t.split_dim[h];
+  if (c == 0) return;
   uint32_t mid = s + c / 2, e = s + c;
   uint32_t base = s + q * LVL_TILE + threadIdx.x * LVL_ITEMS;
   const uint32_t m = t.lmask[h];
@@ -763,11 +780,14 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(ListBufs LB,
     uint32_t id[LVL_ITEMS];
     uint32_t f[LVL_ITEMS];
     uint32_t local = 0;
+    // batches of independent loads, as in k_level_count
+#pragma unroll
+    for (int i = 0; i < LVL_ITEMS; ++i) id[i] = ld[min(base + i, e - 1)];
+#pragma unroll
+    for (int i = 0; i < LVL_ITEMS; ++i) f[i] = bits[id[i] >> 5];
 #pragma unroll
     for (int i = 0; i < LVL_ITEMS; ++i) {
-      uint32_t j = base + i;
-      id[i] = j < e ? ld[j] : 0;
-      f[i] = j < e ? flag_bit(bits, id[i]) : 0;
+      f[i] = base + i < e ? (f[i] >> (id[i] & 31u)) & 1u : 0u;
       local += f[i];
     }
     uint32_t tot;
