@@ -14,10 +14,12 @@ constexpr int SUB_THREADS = 512;
 constexpr int SUB_ITEMS = 8;
 constexpr int SUB_MAX = SUB_THREADS * SUB_ITEMS;  // 4096
 
-// Positions per block in the device-wide level kernels.
+// Positions per block in the device-wide level kernels.  4 per thread: at
+// 10 M bodies a build takes 4.45 ms, against 4.70 (8 per thread), 4.60 (2),
+// 5.27 (1) and 5.13 (16); 128 or 512 threads of 4 were no faster.
 constexpr int LVL_THREADS = 256;
-constexpr int LVL_ITEMS = 8;
-constexpr int LVL_TILE = LVL_THREADS * LVL_ITEMS;  // 2048
+constexpr int LVL_ITEMS = 4;
+constexpr int LVL_TILE = LVL_THREADS * LVL_ITEMS;  // 1024
 
 // Runs of equal sort keys up to this length are ordered by one thread; longer
 // runs are queued (at most n / (FIX_SHORT + 1) of them) for a workgroup each.
