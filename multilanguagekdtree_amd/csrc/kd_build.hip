@@ -982,30 +982,29 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
       if (sgc <= (uint32_t)leaf) {
         sdarr[sgs] = 0xFF;  // leaf: its bodies are finished at this level
       } else {
-        double ext[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          double lo = a.P.c[d][ids[lst[cur][d][sgs]]];
-          double hi = a.P.c[d][ids[lst[cur][d][sgs + sgc - 1]]];
-          ext[d] = hi - lo;
-        }
-        // argmax with ties to the lower dimension, by selects (no indexed
-        // private array)
-        int sd = 0;
-        double best = ext[0];
-        if (ext[1] > best) {
-          sd = 1;
-          best = ext[1];
-        }
-        if (ext[2] > best) {
-          sd = 2;
-          best = ext[2];
-        }
-        uint32_t mid = sgs + sgc / 2;
+        // the ends and the two middle keys of every dimension as one batch of
+        // independent loads (one round trip; the dimension index is a
+        // constant in each load, so no pointer is fetched at run time)
+        const uint32_t mid = sgs + sgc / 2;
+        const uint16_t *l0 = lst[cur][0], *l1 = lst[cur][1], *l2 = lst[cur][2];
+        const double *c0 = a.P.c[0], *c1 = a.P.c[1], *c2 = a.P.c[2];
+        const double e0 = c0[ids[l0[sgs + sgc - 1]]] - c0[ids[l0[sgs]]];
+        const double e1 = c1[ids[l1[sgs + sgc - 1]]] - c1[ids[l1[sgs]]];
+        const double e2 = c2[ids[l2[sgs + sgc - 1]]] - c2[ids[l2[sgs]]];
+        const double m0 = c0[ids[l0[mid]]], p0 = c0[ids[l0[mid - 1]]];
+        const double m1 = c1[ids[l1[mid]]], p1 = c1[ids[l1[mid - 1]]];
+        const double m2 = c2[ids[l2[mid]]], p2 = c2[ids[l2[mid - 1]]];
+        // argmax with ties to the lower dimension, by selects
+        const bool s1 = e1 > e0;
+        const double b01 = s1 ? e1 : e0;
+        const bool s2 = e2 > b01;
+        const int sd = s2 ? 2 : (s1 ? 1 : 0);
+        const double best = s2 ? e2 : b01;
+        const double kv = s2 ? m2 : (s1 ? m1 : m0);
+        const double kprev = s2 ? p2 : (s1 ? p1 : p0);
         uint16_t *ls = lst[cur][sd];
         const double *key = a.P.c[sd];
-        double kv = key[ids[ls[mid]]];
-        if (key[ids[ls[mid - 1]]] == kv) {
+        if (kprev == kv) {
           // equal keys straddle the median: order the run by original index
           uint32_t r0 = mid, r1 = mid + 1;
           while (r0 > sgs && key[ids[ls[r0 - 1]]] == kv) --r0;
