@@ -80,7 +80,9 @@ __device__ __forceinline__ SegV seg_combine(SegV a, SegV b) {
 }
 
 // Exclusive segmented carry for each thread: combination of all earlier
-// threads' aggregates.  shv/shh need NT/64 entries.
+// threads' aggregates.  shv/shh need NT/64 entries.  One barrier: every
+// wavefront combines the per-wavefront totals itself, so the caller must
+// pass a barrier between two calls before shv/shh are written again.
 template <int NT>
 __device__ __forceinline__ SegV block_seg_carry(SegV agg, uint64_t *shv,
                                                 uint32_t *shh) {
@@ -98,23 +100,16 @@ __device__ __forceinline__ SegV block_seg_carry(SegV agg, uint64_t *shv,
     shh[w] = x.h;
   }
   __syncthreads();
-  if (w == 0) {
-    SegV t;
-    t.v = lane < NT / 64 ? shv[lane] : 0;
-    t.h = lane < NT / 64 ? shh[lane] : 0;
+  SegV t;
+  t.v = lane < NT / 64 ? shv[lane] : 0;
+  t.h = lane < NT / 64 ? shh[lane] : 0;
 #pragma unroll
-    for (int o = 1; o < NT / 64; o <<= 1) {
-      SegV y;
-      y.v = __shfl_up(t.v, o, 64);
-      y.h = __shfl_up(t.h, o, 64);
-      if (lane >= o) t = seg_combine(y, t);
-    }
-    if (lane < NT / 64) {
-      shv[lane] = t.v;
-      shh[lane] = t.h;
-    }
+  for (int o = 1; o < NT / 64; o <<= 1) {
+    SegV y;
+    y.v = __shfl_up(t.v, o, 64);
+    y.h = __shfl_up(t.h, o, 64);
+    if (lane >= o) t = seg_combine(y, t);
   }
-  __syncthreads();
   // exclusive within the wave
   SegV prev;
   prev.v = __shfl_up(x.v, 1, 64);
@@ -123,12 +118,15 @@ __device__ __forceinline__ SegV block_seg_carry(SegV agg, uint64_t *shv,
     prev.v = 0;
     prev.h = 0;
   }
+  // the combination of the totals of wavefronts 0 .. w-1
   SegV carry;
-  carry.v = w ? shv[w - 1] : 0;
-  carry.h = w ? shh[w - 1] : 0;
-  SegV r = lane == 0 ? carry : seg_combine(carry, prev);
-  __syncthreads();
-  return r;
+  carry.v = __shfl(t.v, w > 0 ? w - 1 : 0, 64);
+  carry.h = __shfl(t.h, w > 0 ? w - 1 : 0, 64);
+  if (w == 0) {
+    carry.v = 0;
+    carry.h = 0;
+  }
+  return lane == 0 ? carry : seg_combine(carry, prev);
 }
 
 // ---------------------------------------------------------------------------
@@ -914,6 +912,9 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   __shared__ uint8_t sdarr[SUB_MAX];
   __shared__ uint64_t shv[NT / 64];
   __shared__ uint32_t shh[NT / 64];
+  // per level (by parity): some segment of the level splits, so bodies are
+  // still alive below it
+  __shared__ uint32_t splits[2];
 
   const SysDesc S = a.sys[blockIdx.y];
   const uint32_t k = blockIdx.x;
@@ -930,6 +931,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   if (n0 == 0) return;
   const int leaf = a.leaf;
   const int tid = threadIdx.x;
+  if (tid < 2) splits[tid] = 0;
 
   for (uint32_t i = tid; i < n0; i += NT) {
     uint32_t id = L0[sl + i];
@@ -1024,20 +1026,23 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
         t.split_val[h] = kv;
         t.size[h] = best;
         sdarr[sgs] = (uint8_t)sd;
+        splits[lam & 1] = 1;
       }
     }
     __syncthreads();
-    int any = 0;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       if (alive[it] && sdarr[st[it]] == 0xFF) {
         alive[it] = false;
         hl[it] = hbase + cc[it];  // heap slot of the leaf this body ends in
       }
-      any |= alive[it] ? 1 : 0;
     }
     lam_end = lam;
-    if (!__syncthreads_or(any)) break;
+    // a body is still alive iff its segment splits at this level
+    if (!splits[lam & 1]) break;
+    // the other parity's flag was last read at the previous level, before
+    // the barrier above; the next level sets it after later barriers
+    if (tid == 0) splits[(lam + 1) & 1] = 0;
 
     // flags from the split-dim list
 #pragma unroll
