@@ -16,8 +16,6 @@
 // dc/dm_j = (p_j - c) / M, dM/dm_j = 1.  All adjoint sums are FP64 (global
 // FP64 atomics); the mixed path evaluates its terms in FP32 like its forward
 // walk.  Adjoint arrays are in original body order.
-#include <rocprim/warp/warp_reduce.hpp>
-
 #include <algorithm>
 
 #include "kd_internal.hpp"
@@ -26,17 +24,6 @@ namespace kd {
 
 __device__ __forceinline__ uint32_t uni_a(uint32_t x) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)x);
-}
-
-// Sum over the 64 lanes of a wavefront, on every lane: rocprim's DPP
-// reduction (row rotations and broadcasts move the two 32-bit halves in
-// registers; one final cross-lane read), instead of six LDS-routed shuffles.
-__device__ __forceinline__ double wave_sum(double v) {
-  using WR = rocprim::warp_reduce<double, 64>;
-  WR::storage_type st;
-  double out;
-  WR().reduce(v, out, st);
-  return out;
 }
 
 __device__ __forceinline__ uint32_t heap_next_a(uint32_t h, int &depth) {
@@ -86,32 +73,72 @@ struct AdjWalk {
   const double *ux, *uy, *uz;  // seed, original order
   double *lpx, *lpy, *lpz, *lm;  // adjoint accumulators, original order
   double *gx, *gy, *gz, *gh;     // per node: d/dc_n and d/dM_n
+  double *body[4];               // lpx, lpy, lpz, lm
+  double *node[4];               // gx, gy, gz, gh
   uint32_t n;
   uint32_t leaf;
   double theta2;
 };
 
-// Reduce a lane term over the wavefront and add it at index t of four
-// arrays (lane 0 issues the atomics).  `any` is wave-uniform.
-__device__ __forceinline__ void wave_add4(bool any, double a, double b, double c, double d,
-                                          double *pa, double *pb, double *pc, double *pd,
-                                          uint32_t t) {
-  if (!any) return;
-  a = wave_sum(a);
-  b = wave_sum(b);
-  c = wave_sum(c);
-  d = wave_sum(d);
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(pa + t, a);
-    atomicAdd(pb + t, b);
-    atomicAdd(pc + t, c);
-    atomicAdd(pd + t, d);
+// Reaction terms (to a partner body or to a node) are summed over the
+// wavefront before their FP64 atomics.  Instead of four 64-lane reductions
+// per term, a wavefront buffers up to EV terms in LDS, one row of 4 EV values
+// per lane, then lane l sums column l over the 64 rows (in row order) and
+// issues the one atomic for value l: the EV terms' 4 EV atomics go out as
+// one instruction.  The buffer is private to the wavefront and every access
+// happens in wave-uniform control flow; a wavefront's LDS operations execute
+// in order, so only the compiler must not move them (the asm barriers).
+constexpr int EV = 8;
+constexpr int EV_ROW = 4 * EV + 2;  // doubles per lane row; rows stay 16-byte aligned
+constexpr int ADJ_THREADS = 256;
+constexpr uint32_t EV_NODE = 0x80000000u;  // tag bit: node arrays, else body arrays
+
+struct EvBuf {
+  double (*v)[EV_ROW];  // this wavefront's 64 rows
+  uint32_t *tag;        // per buffered term: target index | EV_NODE
+  uint32_t n;           // buffered terms (wave-uniform)
+};
+
+__device__ __forceinline__ void ev_flush(EvBuf &b, const AdjWalk &a) {
+  const uint32_t lane = threadIdx.x & 63;
+  asm volatile("" ::: "memory");
+  if (lane < 4 * b.n) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll 4
+    for (int k = 0; k < 64; k += 4) {
+      s0 += b.v[k][lane];
+      s1 += b.v[k + 1][lane];
+      s2 += b.v[k + 2][lane];
+      s3 += b.v[k + 3][lane];
+    }
+    const double sum = (s0 + s1) + (s2 + s3);
+    const uint32_t tag = b.tag[lane >> 2], c = lane & 3u;
+    double *const *arr = (tag & EV_NODE) ? a.node : a.body;
+    atomicAdd(arr[c] + (tag & ~EV_NODE), sum);
   }
+  asm volatile("" ::: "memory");
+  b.n = 0;
+}
+
+// Add one term (this lane's four values; lanes without a term pass zeros).
+// Called only where some lane has a term (wave-uniform).
+__device__ __forceinline__ void ev_add(EvBuf &b, const AdjWalk &a, double x, double y, double z,
+                                       double w, uint32_t tag) {
+  const uint32_t lane = threadIdx.x & 63;
+  double *row = b.v[lane] + 4 * b.n;
+  row[0] = x;
+  row[1] = y;
+  row[2] = z;
+  row[3] = w;
+  if (lane == 0) b.tag[b.n] = tag;
+  if (++b.n == (uint32_t)EV) ev_flush(b, a);
 }
 
 template <bool MIXED>
-__global__ __launch_bounds__(256) void k_adj_walk(AdjWalk a) {
+__global__ __launch_bounds__(ADJ_THREADS) void k_adj_walk(AdjWalk a) {
 #pragma clang fp contract(off)
+  __shared__ __align__(16) double ev_v[ADJ_THREADS / 64][64][EV_ROW];
+  __shared__ uint32_t ev_t[ADJ_THREADS / 64][EV];
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t wbase = (xcd_block() * blockDim.x + threadIdx.x) & ~63u;
   if (wbase >= a.n) return;
@@ -126,6 +153,10 @@ __global__ __launch_bounds__(256) void k_adj_walk(AdjWalk a) {
   const float uxf = (float)ux, uyf = (float)uy, uzf = (float)uz;
   double sx = 0.0, sy = 0.0, sz = 0.0;
   int D = valid ? 0x7fffffff : -1;
+  EvBuf eb;
+  eb.v = ev_v[threadIdx.x >> 6];
+  eb.tag = ev_t[threadIdx.x >> 6];
+  eb.n = 0;
   uint32_t h = 1;
   int depth = 0;
   while (h != 0) {
@@ -150,8 +181,7 @@ __global__ __launch_bounds__(256) void k_adj_walk(AdjWalk a) {
           sy += gy;
           sz += gz;
         }
-        const uint32_t oj = a.oid[j];
-        wave_add4(__any(use), -gx, -gy, -gz, gm, a.lpx, a.lpy, a.lpz, a.lm, oj);
+        if (__any(use)) ev_add(eb, a, -gx, -gy, -gz, gm, uni_a(a.oid[j]));
       }
       h = heap_next_a(h, depth);
       continue;
@@ -189,7 +219,7 @@ __global__ __launch_bounds__(256) void k_adj_walk(AdjWalk a) {
       sy += gy;
       sz += gz;
     }
-    wave_add4(__any(take), -gx, -gy, -gz, gm, a.gx, a.gy, a.gz, a.gh, h - 1);
+    if (__any(take)) ev_add(eb, a, -gx, -gy, -gz, gm, (h - 1) | EV_NODE);
     D = take ? depth : (open ? 0x7fffffff : D);
     if (__any(open)) {
       h = 2 * h;
@@ -198,6 +228,7 @@ __global__ __launch_bounds__(256) void k_adj_walk(AdjWalk a) {
       h = heap_next_a(h, depth);
     }
   }
+  if (eb.n) ev_flush(eb, a);
   if (valid) {
     atomicAdd(a.lpx + o, sx);
     atomicAdd(a.lpy + o, sy);
@@ -290,13 +321,22 @@ hipError_t launch_adj_step(const AdjStep &st, hipStream_t s) {
   a.gy = st.gnode + st.slots;
   a.gz = st.gnode + 2 * (size_t)st.slots;
   a.gh = st.gnode + 3 * (size_t)st.slots;
+  a.body[0] = a.lpx;
+  a.body[1] = a.lpy;
+  a.body[2] = a.lpz;
+  a.body[3] = a.lm;
+  a.node[0] = a.gx;
+  a.node[1] = a.gy;
+  a.node[2] = a.gz;
+  a.node[3] = a.gh;
   a.n = n;
   a.leaf = st.leaf;
   a.theta2 = st.theta * st.theta;
+  const unsigned wb = (n + ADJ_THREADS - 1) / ADJ_THREADS;
   if (st.mixed)
-    k_adj_walk<true><<<nb, 256, 0, s>>>(a);
+    k_adj_walk<true><<<wb, ADJ_THREADS, 0, s>>>(a);
   else
-    k_adj_walk<false><<<nb, 256, 0, s>>>(a);
+    k_adj_walk<false><<<wb, ADJ_THREADS, 0, s>>>(a);
   k_adj_nodes<<<(st.slots + 255) / 256, 256, 0, s>>>(st.seg_count, st.rec64, st.pos64, st.oid,
                                                      a.gx, a.gy, a.gz, a.gh, st.slots, st.leaf,
                                                      st.lp[0], st.lp[1], st.lp[2], st.lm);
