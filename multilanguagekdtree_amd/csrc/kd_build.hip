@@ -1122,7 +1122,7 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   // order); each body ranks itself among its leaf by original index, using
   // the spare list buffer as staging for those indices, and its state goes
   // straight to its slot in tree order.
-  uint32_t *oidl = reinterpret_cast<uint32_t *>(&lst[cur ^ 1][0][0]);  // 16 KiB
+  uint32_t *oidl = reinterpret_cast<uint32_t *>(&lst[cur ^ 1][0][0]);  // two lists' room
   uint16_t *permloc = &lst[cur ^ 1][2][0];  // tree position -> local index
   uint32_t ow[IT];
 #pragma unroll

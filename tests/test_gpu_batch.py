@@ -11,9 +11,9 @@ from multilanguagekdtree_amd import kdtree as K
 
 pytestmark = pytest.mark.gpu
 
-# uneven systems: one-leaf, a few leaves, subtree-only (<= 4096), and ones
-# that need device-wide levels (> 4096); not multiples of 64
-SIZES = [5, 70, 1000, 4096, 4097, 9000, 30001, 2000, 65, 120000, 1, 777]
+# uneven systems: one-leaf, a few leaves, subtree-only (<= 2048, the subtree
+# builder's bound), and ones that need device-wide levels; not multiples of 64
+SIZES = [5, 70, 1000, 2048, 2049, 4096, 4097, 9000, 30001, 2000, 65, 120000, 1, 777]
 
 
 def single(b, prec, fn):

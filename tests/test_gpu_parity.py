@@ -34,7 +34,8 @@ def lattice(nx, ny, nz, seed=0):
     (1000, 7, K.KD_LAYOUT_COMPACT), (1000, 8, K.KD_LAYOUT_PADDED),
     (1000, 7, K.KD_LAYOUT_PADDED), (1000, 8, K.KD_LAYOUT_COMPACT),
     (1, 7, 0), (2, 7, 0), (7, 7, 0), (8, 7, 0), (12, 7, 0), (15, 7, 0),
-    (4096, 7, 0), (4097, 7, 0), (20000, 7, 0), (100000, 7, 0), (100000, 8, 1),
+    (2048, 7, 0), (2049, 7, 0), (4096, 7, 0), (4097, 7, 0), (20000, 7, 0), (100000, 7, 0),
+    (100000, 8, 1),
     (33333, 2, 0), (33333, 32, 0)])
 def test_tree_matches_oracle(orc, n, leaf, layout):
     b = K.ring(n, seed=n)
