@@ -427,11 +427,13 @@ static uint32_t records_for(const kd_ctx *c) {
   return own | c->extra_recs;
 }
 
-static int build_impl(kd_ctx *c) {
+// range_ready: the last kick-drift left the range of the current state in
+// c->b.range (only within one kd_step call, where nothing else changes it).
+static int build_impl(kd_ctx *c, bool range_ready = false) {
   int nxt = c->cur ^ 1;
   c->tree.recs = records_for(c);
   HIP_TRY(launch_build(c->forest, c->params.leaf_size, c->st[c->cur], c->st[nxt], c->tree, c->b,
-                       c->pos64, c->pos32, c->params.theta, c->stream));
+                       c->pos64, c->pos32, c->params.theta, c->stream, range_ready));
   c->cur = nxt;
   c->has_tree = true;
   return KD_OK;
@@ -524,12 +526,14 @@ int kd_step(kd_ctx *c, int64_t steps) {
   int rc;
   for (int64_t s = 0; s < steps; ++s) {
     if ((rc = record(c, 0))) return rc;
-    if ((rc = build_impl(c))) return rc;
+    // after the first step the previous kick-drift computed the range
+    if ((rc = build_impl(c, s > 0))) return rc;
     if ((rc = record(c, 1))) return rc;
     if ((rc = traverse_impl(c))) return rc;
     if ((rc = record(c, 2))) return rc;
     HIP_TRY(launch_kick_drift(c->st[c->cur], c->acc[0], c->acc[1], c->acc[2], (uint32_t)c->n,
-                              c->params.dt, c->bad, c->stream));
+                              c->params.dt, c->bad, c->stream,
+                              s + 1 < steps ? c->b.range : nullptr));
     if ((rc = record(c, 3))) return rc;
   }
   if (c->params.profile && (rc = drain_timings(c))) return rc;

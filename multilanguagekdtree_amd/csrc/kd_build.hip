@@ -28,12 +28,6 @@
 
 namespace kd {
 
-__device__ __forceinline__ uint64_t ordkey(double v) {
-  uint64_t b = (uint64_t)__double_as_longlong(v);
-  if ((b << 1) == 0) b = 0;  // -0.0 and +0.0 compare equal
-  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
-}
-
 // ---------------------------------------------------------------------------
 // block scans
 // ---------------------------------------------------------------------------
@@ -192,11 +186,10 @@ __global__ void k_range_init(unsigned long long *range) {
   }
 }
 
-// One block reduces its share to one minimum and one maximum per dimension
-// (wavefront shuffles, then LDS over its four wavefronts); only thread d of
-// each block updates the global range, so the atomics on the six words number
-// 6 per block rather than 6 per wavefront.
-constexpr uint32_t RANGE_BLOCKS = 1024;
+hipError_t launch_range_reset(unsigned long long *range, hipStream_t s) {
+  k_range_init<<<1, 64, 0, s>>>(range);
+  return hipGetLastError();
+}
 
 __global__ __launch_bounds__(256) void k_range(const double *__restrict__ x,
                                                const double *__restrict__ y,
@@ -225,36 +218,7 @@ __global__ __launch_bounds__(256) void k_range(const double *__restrict__ x,
       hi[d] = k[d] > hi[d] ? k[d] : hi[d];
     }
   }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const uint64_t a = __shfl_xor(lo[d], o, 64), b = __shfl_xor(hi[d], o, 64);
-      lo[d] = a < lo[d] ? a : lo[d];
-      hi[d] = b > hi[d] ? b : hi[d];
-    }
-  }
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      sh[2 * d][w] = lo[d];
-      sh[2 * d + 1][w] = hi[d];
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x < 6) {
-    const int v = threadIdx.x;
-    uint64_t r = sh[v][0];
-    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
-      const uint64_t c = sh[v][q];
-      r = (v & 1) ? (c > r ? c : r) : (c < r ? c : r);
-    }
-    if (v & 1)
-      atomicMax(&range[v], (unsigned long long)r);
-    else
-      atomicMin(&range[v], (unsigned long long)r);
-  }
+  range_commit(lo, hi, sh, range);
 }
 
 // Body keys span [0, PAD_KEY - 1]; in a forest, the padding positions that
@@ -1346,7 +1310,7 @@ hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t
 
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
-                        double theta, hipStream_t s) {
+                        double theta, hipStream_t s, bool range_ready) {
   const double *coord[3] = {in.x, in.y, in.z};
   const unsigned ny = (unsigned)f.nsys;
   hipError_t e;
@@ -1355,9 +1319,11 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   // in coordinate order at [list_off, list_off + n) (padding last)
   const uint32_t nsort = f.nsys == 1 ? f.max_n : f.total;
   if (sort_temp_bytes(nsort, f.nsys) > b.sort_tmp_bytes) return hipErrorInvalidValue;
-  k_range_init<<<1, 64, 0, s>>>(b.range);
-  k_range<<<std::min<uint32_t>((nsort + 255) / 256, RANGE_BLOCKS), 256, 0, s>>>(
-      in.x, in.y, in.z, nsort, b.range);
+  if (!range_ready) {
+    k_range_init<<<1, 64, 0, s>>>(b.range);
+    k_range<<<std::min<uint32_t>((nsort + 255) / 256, RANGE_BLOCKS), 256, 0, s>>>(
+        in.x, in.y, in.z, nsort, b.range);
+  }
   // the three coordinate sorts are independent: dimension 0 runs on s,
   // dimensions 1 and 2 on the side streams, forked after the range and
   // joined before the levels (a graph capture of s follows the fork)

@@ -40,6 +40,55 @@ __device__ __forceinline__ uint32_t xcd_block() {
   const uint32_t x = blockIdx.x % 8u, i = blockIdx.x / 8u;
   return x * q + (x < r ? x : r) + i;
 }
+
+// Order-preserving 64-bit key of a double; -0.0 and +0.0 share one key.
+__device__ __forceinline__ uint64_t ordkey(double v) {
+  uint64_t b = (uint64_t)__double_as_longlong(v);
+  if ((b << 1) == 0) b = 0;
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+// Coordinate range for the build's sort keys: range[2d], range[2d+1] hold the
+// ordered-key minimum and maximum of dimension d.  A kernel that produces it
+// reduces each block's share (wavefront shuffles, then LDS over the block's
+// wavefronts, sh holding 6 x (blockDim.x / 64) entries) and only threads 0-5
+// of a block update the six words; grids are capped at RANGE_BLOCKS blocks,
+// so the atomics stay few.  Every thread of the block must call this.
+constexpr uint32_t RANGE_BLOCKS = 1024;
+__device__ __forceinline__ void range_commit(uint64_t lo[3], uint64_t hi[3], uint64_t (*sh)[4],
+                                             unsigned long long *range) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const uint64_t a = __shfl_xor(lo[d], o, 64), b = __shfl_xor(hi[d], o, 64);
+      lo[d] = a < lo[d] ? a : lo[d];
+      hi[d] = b > hi[d] ? b : hi[d];
+    }
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      sh[2 * d][w] = lo[d];
+      sh[2 * d + 1][w] = hi[d];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int v = threadIdx.x;
+    uint64_t r = sh[v][0];
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+      const uint64_t c = sh[v][q];
+      r = (v & 1) ? (c > r ? c : r) : (c < r ? c : r);
+    }
+    if (v & 1)
+      atomicMax(&range[v], (unsigned long long)r);
+    else
+      atomicMin(&range[v], (unsigned long long)r);
+  }
+}
+
 // Tree-ordered body arrays carry this many padding entries so that batched
 // leaf loads (up to 8 bodies) never leave the allocation.
 constexpr int BODY_PAD = 32;
@@ -183,9 +232,11 @@ size_t sort_temp_bytes(uint32_t n, int nsys);
 size_t level_cnt_entries(const Forest &f);
 size_t tieflag_entries(const Forest &f);
 hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t s);
+// range_ready: b.range already holds the range of `in` (computed by the
+// kick-drift that produced it), so the build skips its own range pass.
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
-                        double theta, hipStream_t s);
+                        double theta, hipStream_t s, bool range_ready = false);
 hipError_t launch_traverse(int precision, bool count, int variant, const TreeDev &t,
                            const double4 *pos64, const float4 *pos32, uint32_t n,
                            double theta, int leaf, double *ax, double *ay, double *az,
@@ -228,9 +279,12 @@ hipError_t launch_select(int64_t *idx, const double *vals, int64_t n, int64_t nv
 // Replace every memset node of a captured graph by a fill kernel node with
 // the same dependencies (memset nodes are not replayed reliably here).
 hipError_t graph_replace_memsets(hipGraph_t g, int *replaced);
+// range (may be null): also computes the next build's coordinate range of
+// the updated positions (reset first), so that build can skip its own pass.
 hipError_t launch_kick_drift(const StateDev &s, const double *ax, const double *ay,
                              const double *az, uint32_t n, double dt, int *bad,
-                             hipStream_t st);
+                             hipStream_t st, unsigned long long *range = nullptr);
+hipError_t launch_range_reset(unsigned long long *range, hipStream_t s);
 hipError_t launch_to_original(const uint32_t *oid, const double *a0, const double *a1,
                               const double *a2, double *o0, double *o1, double *o2,
                               uint32_t n, hipStream_t st);

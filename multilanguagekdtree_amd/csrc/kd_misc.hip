@@ -7,27 +7,42 @@
 namespace kd {
 
 // Kick-drift, Parallel/RustVersion/src/array_kd_tree.rs:649-662 (semi-implicit
-// Euler, unfused FP64).  Flags a non-finite result in *bad.
-__global__ void k_kick_drift(StateDev s, const double *__restrict__ ax,
-                             const double *__restrict__ ay,
-                             const double *__restrict__ az, uint32_t n, double dt,
-                             int *bad) {
+// Euler, unfused FP64).  Flags a non-finite result in *bad.  With range set,
+// also reduces the new positions' coordinate range for the next build (the
+// grid is then capped at RANGE_BLOCKS blocks and strides over the bodies).
+__global__ __launch_bounds__(256) void k_kick_drift(StateDev s, const double *__restrict__ ax,
+                                                    const double *__restrict__ ay,
+                                                    const double *__restrict__ az, uint32_t n,
+                                                    double dt, int *bad,
+                                                    unsigned long long *range) {
 #pragma clang fp contract(off)
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double vx = s.vx[i] + dt * ax[i];
-  double vy = s.vy[i] + dt * ay[i];
-  double vz = s.vz[i] + dt * az[i];
-  double x = s.x[i] + dt * vx;
-  double y = s.y[i] + dt * vy;
-  double z = s.z[i] + dt * vz;
-  s.vx[i] = vx;
-  s.vy[i] = vy;
-  s.vz[i] = vz;
-  s.x[i] = x;
-  s.y[i] = y;
-  s.z[i] = z;
-  if (!(isfinite(x) && isfinite(y) && isfinite(z))) *bad = 1;
+  __shared__ uint64_t sh[6][4];
+  uint64_t lo[3] = {~0ull, ~0ull, ~0ull}, hi[3] = {0ull, 0ull, 0ull};
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double vx = s.vx[i] + dt * ax[i];
+    double vy = s.vy[i] + dt * ay[i];
+    double vz = s.vz[i] + dt * az[i];
+    double x = s.x[i] + dt * vx;
+    double y = s.y[i] + dt * vy;
+    double z = s.z[i] + dt * vz;
+    s.vx[i] = vx;
+    s.vy[i] = vy;
+    s.vz[i] = vz;
+    s.x[i] = x;
+    s.y[i] = y;
+    s.z[i] = z;
+    if (!(isfinite(x) && isfinite(y) && isfinite(z))) *bad = 1;
+    if (range) {
+      const uint64_t k[3] = {ordkey(x), ordkey(y), ordkey(z)};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = k[d] < lo[d] ? k[d] : lo[d];
+        hi[d] = k[d] > hi[d] ? k[d] : hi[d];
+      }
+    }
+  }
+  if (range) range_commit(lo, hi, sh, range);  // range is the same for the whole grid
 }
 
 // out[oid[t]] = a[t] for three components.
@@ -114,8 +129,14 @@ __global__ __launch_bounds__(256) void k_direct(const double *__restrict__ x,
 
 hipError_t launch_kick_drift(const StateDev &s, const double *ax, const double *ay,
                              const double *az, uint32_t n, double dt, int *bad,
-                             hipStream_t st) {
-  k_kick_drift<<<(n + 255) / 256, 256, 0, st>>>(s, ax, ay, az, n, dt, bad);
+                             hipStream_t st, unsigned long long *range) {
+  uint32_t blocks = (n + 255) / 256;
+  if (range) {
+    hipError_t e = launch_range_reset(range, st);
+    if (e != hipSuccess) return e;
+    blocks = std::min<uint32_t>(blocks, RANGE_BLOCKS);
+  }
+  k_kick_drift<<<blocks, 256, 0, st>>>(s, ax, ay, az, n, dt, bad, range);
   return hipGetLastError();
 }
 
