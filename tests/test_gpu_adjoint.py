@@ -45,9 +45,9 @@ def reference_positions(b, dt, steps, theta, seed=1):
     return P(pr)
 
 
-def gpu_grads(b, pref, steps, every, theta, dt, prec):
+def gpu_grads(b, pref, steps, every, theta, dt, prec, leaf=7):
     n = len(b["x"])
-    sim = Adjoint(n, steps, every, theta=theta, dt=dt, precision=prec)
+    sim = Adjoint(n, steps, every, theta=theta, dt=dt, leaf_size=leaf, precision=prec)
     pN, vN = sim.forward(cuda(P(b)), cuda(V(b)), cuda(b["m"]))
     gp = 2 * (pN - cuda(pref))
     gx, gv, gm = sim.backward(gp, None)
@@ -55,8 +55,8 @@ def gpu_grads(b, pref, steps, every, theta, dt, prec):
     return gx.cpu().numpy(), gv.cpu().numpy(), gm.cpu().numpy(), pN.cpu().numpy()
 
 
-def oracle_grads(b, pref, steps, theta, dt):
-    fin, tape = oracle.adj_forward(b, dt, steps, theta)
+def oracle_grads(b, pref, steps, theta, dt, leaf=7):
+    fin, tape = oracle.adj_forward(b, dt, steps, theta, leaf=leaf)
     return oracle.adj_backward(tape, 2 * (P(fin) - pref), np.zeros((len(b["x"]), 3)))
 
 
@@ -83,6 +83,19 @@ def test_fp64_gradient_matches_oracle_far_field():
     ox, ov, om = oracle_grads(b, pref, 8, 0.5, 1e-2)
     for g, o in ((gx, ox), (gv, ov), (gm, om)):
         assert rel(g, o) <= 1e-9, rel(g, o)
+
+
+@pytest.mark.parametrize("leaf", [2, 32])
+def test_fp64_gradient_matches_oracle_leaf_sizes(leaf):
+    # leaf 32: up to 32 partner terms per leaf, so a wavefront's buffer of
+    # reaction terms is summed and flushed several times inside one leaf;
+    # leaf 2: mostly node terms
+    b = K.ring(2000, seed=25)
+    pref = reference_positions(b, 1e-3, 10, 0.3)
+    gx, gv, gm, _ = gpu_grads(b, pref, 10, 5, 0.3, 1e-3, K.KD_PREC_FP64, leaf=leaf)
+    ox, ov, om = oracle_grads(b, pref, 10, 0.3, 1e-3, leaf=leaf)
+    for g, o in ((gx, ox), (gv, ov), (gm, om)):
+        assert rel(g, o) <= 1e-9, (leaf, rel(g, o))
 
 
 def test_mixed_gradient_close_to_fp64_reference():
