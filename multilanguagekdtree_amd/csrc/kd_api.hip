@@ -977,7 +977,7 @@ struct kd_batch {
   int64_t n_total = 0;              // sum of n (the caller's concatenated layout)
   uint2 *tasks = nullptr;           // (system, first body) per wavefront, largest systems first
   uint32_t ntasks = 0;
-  uint32_t *queue = nullptr;        // task counter of the persistent walk
+  uint32_t *queue = nullptr;        // task counters of the persistent walk, one per XCD
   int cus = 0;
   WalkShape ws;
   bool use_graph = true;
@@ -1165,7 +1165,7 @@ extern "C" int kd_batch_create(int64_t nsys, const int64_t *n, const kd_params *
     for (int64_t w = 0; w < n[i]; w += 64) tasks.push_back(make_uint2((uint32_t)i, (uint32_t)w));
   b->ntasks = (uint32_t)tasks.size();
   if (hipMalloc(&b->tasks, sizeof(uint2) * tasks.size()) != hipSuccess ||
-      hipMalloc(&b->queue, sizeof(uint32_t)) != hipSuccess) {
+      hipMalloc(&b->queue, 8 * sizeof(uint32_t)) != hipSuccess) {
     fail(KD_ERR_NOMEM, "task table allocation failed");
     return bail(KD_ERR_NOMEM);
   }
