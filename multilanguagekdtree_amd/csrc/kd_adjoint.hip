@@ -86,8 +86,9 @@ struct AdjWalk {
 // per lane, then lane l sums column l over the 64 rows (in row order) and
 // issues the one atomic for value l: the EV terms' 4 EV atomics go out as
 // one instruction.  The buffer is private to the wavefront and every access
-// happens in wave-uniform control flow; a wavefront's LDS operations execute
-// in order, so only the compiler must not move them (the asm barriers).
+// happens in wave-uniform control flow, so no workgroup barrier is needed:
+// the flush waits for the wavefront's own LDS writes (lgkmcnt 0) before the
+// column reads, and the asm barriers keep the compiler from moving them.
 constexpr int EV = 8;
 constexpr int EV_ROW = 4 * EV + 2;  // doubles per lane row; rows stay 16-byte aligned
 constexpr int ADJ_THREADS = 256;
@@ -102,6 +103,7 @@ struct EvBuf {
 __device__ __forceinline__ void ev_flush(EvBuf &b, const AdjWalk &a) {
   const uint32_t lane = threadIdx.x & 63;
   asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt left at their maximum
   if (lane < 4 * b.n) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll 4
