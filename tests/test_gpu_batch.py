@@ -113,6 +113,21 @@ def test_autotune_keeps_results(bodies):
         np.testing.assert_array_equal(bt.accel(), ref)
 
 
+@pytest.mark.parametrize("threads,waves", [(64, 8), (256, 32), (512, 16), (128, 1)])
+def test_persistent_walk_matches_one_launch_per_task(bodies, threads, waves):
+    # the persistent walk takes tasks from one counter per XCD (chunks dealt
+    # round robin); every task must run exactly once, so the accelerations
+    # equal those of one wavefront per task bit for bit
+    with K.Batch(SIZES) as bt:
+        bt.upload(bodies)
+        bt.set_tuning("walk_threads", 64)
+        bt.set_tuning("walk_waves_per_cu", 0)
+        ref = bt.accel()
+        bt.set_tuning("walk_threads", threads)
+        bt.set_tuning("walk_waves_per_cu", waves)
+        np.testing.assert_array_equal(bt.accel(), ref)
+
+
 def test_counters_sum_over_systems(bodies):
     with K.Batch(SIZES, count_interactions=True) as bt:
         bt.upload(bodies)
