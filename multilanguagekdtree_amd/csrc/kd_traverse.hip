@@ -295,17 +295,19 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
   const bool valid = me < a.n;
   const typename P::Self p = P::self(bodies[valid ? me : wbase]);
   typename P::Acc acc;
-  // levels count from 1 at the root, so D = 0 keeps a lane out of every node
-  // and ~0u (while it keeps opening) lets it into every node
-  uint32_t D = valid ? ~0u : 0u;
+  // the lane's acceptance level D (root 1; unbounded while it keeps opening)
+  // is kept as E = 32 - D and compared with clz(h) = 32 - level(h), which
+  // saves forming the level: E = 32 keeps a lane out of every node, E = 0
+  // lets it into every node
+  int E = valid ? 0 : 32;
   unsigned long long nt = 0, na = 0, np = 0, wv = 0, wl = 0;
   uint32_t h = 1;
   do {
     const typename P::Rec r = load_rec(nodes_m1, uni(h));
-    const uint32_t level = 32 - __builtin_clz(h);
+    const int lz = __builtin_clz(h);  // 32 - level(h)
     // the lanes taking part, as a scalar mask (kept on the scalar unit) and
     // as the lane's own condition
-    const uint64_t actm = __builtin_amdgcn_ballot_w64(level <= D);
+    const uint64_t actm = __builtin_amdgcn_ballot_w64(lz >= E);
     const bool active = __builtin_amdgcn_inverse_ballot_w64(actm);
     if (COUNT) ++wv;
     if (r.count <= a.leaf) {
@@ -327,8 +329,9 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
       const bool open = __builtin_amdgcn_inverse_ballot_w64(om);
       // wave-uniform: the far-field term is evaluated only when a lane takes it
       if (actm & okm) P::m2p_sel(acc, r, f, take);
-      // a taking lane had D >= level, a lane outside D < level
-      D = open ? ~0u : min(D, level);
+      // D = open ? unbounded : min(D, level): a taking lane had D >= level,
+      // a lane outside D < level
+      E = open ? 0 : max(E, lz);
       if (COUNT) {
         nt += active ? 1 : 0;
         na += take ? 1 : 0;
