@@ -86,7 +86,10 @@ int kd_default_params(kd_params *out);
 
 /* Context with capacity for cap_n bodies on the current HIP device.
  * Replaces the per-call allocations of simple_sim (acc, tree, indices:
- * Parallel/RustVersion/src/array_kd_tree.rs:624-630). */
+ * Parallel/RustVersion/src/array_kd_tree.rs:624-630).  KD_ERR_CAPACITY when
+ * the tree of cap_n bodies at params->leaf_size would exceed 2^26 - 1 heap
+ * slots (about leaf_size * 2^25 bodies); kd_set_params and kd_batch_create
+ * (per system) check the same limit. */
 int kd_create(int64_t cap_n, const kd_params *params, kd_ctx **out);
 int kd_destroy(kd_ctx *ctx);
 int kd_set_params(kd_ctx *ctx, const kd_params *params);

@@ -111,6 +111,22 @@ static int check_params(const kd_params *p) {
   return KD_OK;
 }
 
+// The walks address a node record at the 32-bit byte offset h * (record
+// size), one scalar instruction per visit cheaper than a 64-bit offset; with
+// records of up to 64 bytes one tree may hold at most 2^26 - 1 heap slots
+// (about leaf_size * 2^25 bodies: 235 M at leaf size 7, 1.07 G at 32).
+constexpr int64_t MAX_TREE_SLOTS = ((int64_t)1 << 26) - 1;
+
+static int check_tree_size(int64_t n, int leaf) {
+  const Shape sh = make_shape(n, leaf);
+  if (sh.slots > MAX_TREE_SLOTS)
+    return fail(KD_ERR_CAPACITY,
+                "a tree of %lld bodies at leaf_size %d has %lld heap slots, above the walk's "
+                "limit of 2^26 - 1 (raise leaf_size, or split the system)",
+                (long long)n, leaf, (long long)sh.slots);
+  return KD_OK;
+}
+
 static int alloc_nodes(kd_ctx *c, int64_t s) {
   if (s > c->slots_alloc) {
     for (void *p : c->node_allocs) (void)hipFree(p);
@@ -298,6 +314,7 @@ int kd_create(int64_t cap_n, const kd_params *params, kd_ctx **out) {
   }
   int rc = check_params(&p);
   if (rc) return rc;
+  if ((rc = check_tree_size(cap_n, p.leaf_size))) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(KD_ERR_NODEVICE, "no HIP device visible");
@@ -325,6 +342,7 @@ int kd_set_params(kd_ctx *c, const kd_params *p) {
   if (!c) return fail(KD_ERR_ARG, "ctx is NULL");
   int rc = check_params(p);
   if (rc) return rc;
+  if ((rc = check_tree_size(c->cap, p->leaf_size))) return rc;
   bool leaf_changed = p->leaf_size != c->params.leaf_size;
   c->params = *p;
   if (leaf_changed) c->has_tree = false;
@@ -1113,6 +1131,7 @@ extern "C" int kd_batch_create(int64_t nsys, const int64_t *n, const kd_params *
   int64_t total_n = 0;
   for (int64_t i = 0; i < nsys; ++i) {
     if (n[i] < 1 || n[i] > 0x7fffffffLL) return fail(KD_ERR_ARG, "n[%lld] out of range", (long long)i);
+    if ((rc = check_tree_size(n[i], p.leaf_size))) return rc;
     const Shape sh = make_shape(n[i], p.leaf_size);
     hs[i] = make_sys(sh, (uint32_t)body, (uint32_t)slots, (uint32_t)total_n);
     body += ((uint64_t)n[i] + 63) & ~63ull;

@@ -93,6 +93,21 @@ def test_bad_arguments_rejected_on_host():
     assert L.kd_step(None, 1) == _lib.KD_ERR_ARG
 
 
+def test_tree_size_limit_rejected_on_host():
+    # one tree holds at most 2^26 - 1 heap slots (32-bit record offsets in the
+    # walk): 300 M bodies at leaf size 7, or 100 M at leaf size 2, are refused
+    # before any device work; 230 M at leaf size 7 passes that check
+    L = _lib.lib()
+    import ctypes as C
+    ctx = C.c_void_p()
+    for n, leaf in ((300_000_000, 7), (100_000_000, 2)):
+        p = _lib.default_params(leaf_size=leaf)
+        assert L.kd_create(n, C.byref(p), C.byref(ctx)) == _lib.KD_ERR_CAPACITY
+        assert b"heap slots" in L.kd_last_error()
+    p = _lib.default_params(leaf_size=7)
+    assert L.kd_create(230_000_000, C.byref(p), C.byref(ctx)) != _lib.KD_ERR_CAPACITY
+
+
 def _chain(n):
     nn = 2 * n - 1
     t = K.Tree(is_leaf=np.zeros(nn, np.int32), split_dim=np.zeros(nn, np.int32),
