@@ -788,6 +788,8 @@ struct SubArgs {
 };
 
 // Heap index (1-based) of the node after the subtree of heap slot h (0-based).
+// Records store it times their own size: the byte offset at which the walk
+// loads the next record (below 2^32 by the 2^26 heap-slot limit).
 __device__ __forceinline__ uint32_t skip_of(uint32_t h) {
   uint32_t x = h + 1;
   x >>= __builtin_ctz(~x);
@@ -801,7 +803,7 @@ __device__ void write_leaf_rec(const TreeDev &t, uint32_t h, uint32_t start,
   r.cx = r.cy = r.cz = r.m = r.size2 = 0.0;
   r.start = start;
   r.count = count;
-  r.skip = skip;
+  r.skip = skip * (uint32_t)sizeof(NodeRec64);
   r.pad0 = 0;
   r.pad1 = 0.0;
   if (t.recs & REC_64) t.rec64[h] = r;
@@ -809,14 +811,14 @@ __device__ void write_leaf_rec(const TreeDev &t, uint32_t h, uint32_t start,
   q.cx = q.cy = q.cz = q.m = q.size2 = 0.f;
   q.start = start;
   q.count = count;
-  q.skip = skip;
+  q.skip = skip * (uint32_t)sizeof(NodeRec32);
   if (t.recs & REC_32) t.rec32[h] = q;
   NodeRecD dd;
   dd.cx = dd.cy = dd.cz = 0.0;
   dd.m = dd.size2 = 0.f;
   dd.start = start;
   dd.count = count;
-  dd.skip = skip;
+  dd.skip = skip * (uint32_t)sizeof(NodeRecD);
   dd.pad0 = 0;
   dd.pad1 = 0.0;
   if (t.recs & REC_D) t.recd[h] = dd;
@@ -839,7 +841,7 @@ __device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
   r.size2 = s2;
   r.start = start;
   r.count = count;
-  r.skip = skip;
+  r.skip = skip * (uint32_t)sizeof(NodeRec64);
   r.pad0 = 0;
   r.pad1 = 0.0;
   if (t.recs & REC_64) t.rec64[h] = r;
@@ -851,7 +853,7 @@ __device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
   q.size2 = open2;
   q.start = start;
   q.count = count;
-  q.skip = skip;
+  q.skip = skip * (uint32_t)sizeof(NodeRec32);
   if (t.recs & REC_32) t.rec32[h] = q;
   NodeRecD dd;
   dd.cx = cx;
@@ -861,7 +863,7 @@ __device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
   dd.size2 = open2;
   dd.start = start;
   dd.count = count;
-  dd.skip = skip;
+  dd.skip = skip * (uint32_t)sizeof(NodeRecD);
   dd.pad0 = 0;
   dd.pad1 = 0.0;
   if (t.recs & REC_D) t.recd[h] = dd;

@@ -45,12 +45,12 @@ __device__ __forceinline__ uint32_t uni(uint32_t x) {
 // One node record as a single wide value (a vector of 32-bit words), so that
 // the scalar unit fetches it with one load instead of per-field loads.
 // nodes_m1 points one record before the first, so the 1-based heap index h
-// is a plain 32-bit byte offset h*sizeof(Rec).
+// is the byte offset h * sizeof(Rec).
 template <class Rec>
-__device__ __forceinline__ Rec load_rec(const Rec *__restrict__ nodes_m1, uint32_t h) {
+__device__ __forceinline__ Rec load_rec_at(const Rec *__restrict__ nodes_m1, uint32_t off) {
   typedef uint32_t words __attribute__((ext_vector_type(sizeof(Rec) / 4)));
   const char *base = reinterpret_cast<const char *>(nodes_m1);
-  const words w = *reinterpret_cast<const words *>(base + (h * (uint32_t)sizeof(Rec)));
+  const words w = *reinterpret_cast<const words *>(base + off);
   Rec r;
   __builtin_memcpy(&r, &w, sizeof(Rec));
   return r;
@@ -295,16 +295,22 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
   const bool valid = me < a.n;
   const typename P::Self p = P::self(bodies[valid ? me : wbase]);
   typename P::Acc acc;
-  // the lane's acceptance level D (root 1; unbounded while it keeps opening)
-  // is kept as E = 32 - D and compared with clz(h) = 32 - level(h), which
-  // saves forming the level: E = 32 keeps a lane out of every node, E = 0
-  // lets it into every node
-  int E = valid ? 0 : 32;
+  // The position is the byte offset of the current record, off = h RS (the
+  // records' skip pointers are stored the same way), so no multiply is
+  // needed per visit.  The lane's acceptance level D (root 1; unbounded
+  // while it keeps opening) is kept as F = 32 - D - log2(RS) and compared
+  // with clz(off) = 32 - level(h) - log2(RS), which saves forming the level:
+  // F = 32 - log2(RS) keeps a lane out of every node, F = -log2(RS) lets it
+  // into every node.
+  constexpr uint32_t RS = sizeof(typename P::Rec);
+  static_assert(RS == 32 || RS == 64, "records of 32 or 64 bytes");
+  constexpr int RSL = RS == 32 ? 5 : 6;
+  int E = valid ? -RSL : 32 - RSL;
   unsigned long long nt = 0, na = 0, np = 0, wv = 0, wl = 0;
-  uint32_t h = 1;
+  uint32_t h = RS;  // the root, heap index 1
   do {
-    const typename P::Rec r = load_rec(nodes_m1, uni(h));
-    const int lz = __builtin_clz(h);  // 32 - level(h)
+    const typename P::Rec r = load_rec_at(nodes_m1, uni(h));
+    const int lz = __builtin_clz(h);  // 32 - level(h) - log2(RS)
     // the lanes taking part, as a scalar mask (kept on the scalar unit) and
     // as the lane's own condition
     const uint64_t actm = __builtin_amdgcn_ballot_w64(lz >= E);
@@ -331,7 +337,7 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
       if (actm & okm) P::m2p_sel(acc, r, f, take);
       // D = open ? unbounded : min(D, level): a taking lane had D >= level,
       // a lane outside D < level
-      E = open ? 0 : max(E, lz);
+      E = open ? -RSL : max(E, lz);
       if (COUNT) {
         nt += active ? 1 : 0;
         na += take ? 1 : 0;
