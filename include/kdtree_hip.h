@@ -261,7 +261,9 @@ int kd_adj_forward(kd_adj *a, const double *x0, const double *y0, const double *
                    const double *m, double *x, double *y, double *z, double *vx,
                    double *vy, double *vz);
 /* Given dL/dp_final (gp*) and dL/dv_final (gv*; NULL = 0), the gradients
- * dL/dp0, dL/dv0 and dL/dm (any output may be NULL). */
+ * dL/dp0, dL/dv0 and dL/dm (any output may be NULL) of the LAST completed
+ * kd_adj_forward: a context keeps one run's checkpoints.  A forward that
+ * fails part way leaves none, and this call then returns KD_ERR_STATE. */
 int kd_adj_backward(kd_adj *a, const double *gpx, const double *gpy, const double *gpz,
                     const double *gvx, const double *gvy, const double *gvz,
                     double *gx0, double *gy0, double *gz0, double *gvx0,

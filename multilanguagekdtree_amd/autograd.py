@@ -29,6 +29,10 @@ class Adjoint:
         check(self._L.kd_adj_create(n, steps, checkpoint_every, C.byref(p), C.byref(h)))
         self._h = h
         self.n, self.steps = n, steps
+        # one context holds the checkpoints of its latest forward only; every
+        # forward (successful or not) starts a new generation, and a backward
+        # recorded against an older one is refused
+        self.generation = 0
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -52,6 +56,7 @@ class Adjoint:
         ins = self._cols(x0) + self._cols(v0) + [m.contiguous()]
         outs = [torch.empty(n, dtype=torch.float64, device=x0.device) for _ in range(6)]
         torch.cuda.current_stream().synchronize()
+        self.generation += 1
         check(self._L.kd_adj_forward(self._h, *[t.data_ptr() for t in ins],
                                      *[t.data_ptr() for t in outs]))
         return torch.stack(outs[:3], 1), torch.stack(outs[3:], 1)
@@ -73,10 +78,17 @@ class _Simulate(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x0, v0, m, sim):
         ctx.sim = sim
-        return sim.forward(x0, v0, m)
+        out = sim.forward(x0, v0, m)
+        ctx.generation = sim.generation
+        return out
 
     @staticmethod
     def backward(ctx, gp, gv):
+        if ctx.sim.generation != ctx.generation:
+            raise RuntimeError(
+                "this Adjoint context ran another forward after the simulate() call being "
+                "differentiated, so its checkpoints belong to the later run; use one Adjoint "
+                "per differentiated simulate() call")
         gx, gvv, gm = ctx.sim.backward(gp, gv)
         return gx, gvv, gm, None
 

@@ -1528,6 +1528,9 @@ extern "C" int kd_adj_forward(kd_adj *a, const double *x0, const double *y0, con
                               const double *m, double *x, double *y, double *z, double *vx,
                               double *vy, double *vz) {
   if (!a) return fail(KD_ERR_ARG, "adjoint context is NULL");
+  // the checkpoints are overwritten from here on: until this run completes,
+  // no reverse sweep may read them
+  a->forward_done = false;
   kd_ctx *c = a->c;
   int rc = kd_upload_device(c, a->n, x0, y0, z0, vx0, vy0, vz0, m);
   if (rc) return rc;

@@ -158,3 +158,114 @@ def test_torch_autograd():
     gx, gv, gm = sim.backward(2 * (pN.detach() - pref), vN.detach())
     for a, e in ((x0.grad, gx), (v0.grad, gv), (m.grad, gm)):
         assert rel(a.cpu().numpy(), e.cpu().numpy()) <= 1e-12
+
+
+def test_second_forward_refuses_stale_backward():
+    # one context keeps the checkpoints of its latest forward only: a
+    # backward through an earlier simulate() call must fail, not return the
+    # later run's gradients
+    n, steps = 1000, 4
+    b = K.ring(n, seed=26)
+    sim = Adjoint(n, steps, 2, precision=K.KD_PREC_FP64)
+    v0 = cuda(V(b)).requires_grad_(True)
+    p1, _ = simulate(cuda(P(b)), v0, cuda(b["m"]), sim)
+    p2, _ = simulate(cuda(P(b)), cuda(V(b) * 1.01), cuda(b["m"]), sim)
+    with pytest.raises(RuntimeError, match="another forward"):
+        (p1 ** 2).sum().backward()
+    # the latest call still differentiates
+    v2 = cuda(V(b)).requires_grad_(True)
+    p3, _ = simulate(cuda(P(b)), v2, cuda(b["m"]), sim)
+    (p3 ** 2).sum().backward()
+    assert torch.isfinite(v2.grad).all()
+    sim.close()
+
+
+def test_failed_forward_leaves_no_checkpoints():
+    # a forward that fails (non-finite input, refused by the upload check)
+    # clears the completed-run flag, so the reverse sweep reports the state
+    # error instead of reading half-overwritten checkpoints
+    n, steps = 1000, 4
+    b = K.ring(n, seed=27)
+    sim = Adjoint(n, steps, 2, precision=K.KD_PREC_FP64)
+    sim.forward(cuda(P(b)), cuda(V(b)), cuda(b["m"]))
+    bad = P(b).copy()
+    bad[5, 0] = np.nan
+    with pytest.raises(RuntimeError):
+        sim.forward(cuda(bad), cuda(V(b)), cuda(b["m"]))
+    with pytest.raises(RuntimeError, match="has not run"):
+        sim.backward(cuda(np.ones((n, 3))), None)
+    sim.close()
+
+
+def gpu_final(b, steps, dt, prec, theta=0.3):
+    with K.Simulation(len(b["x"]), theta=theta, dt=dt, precision=prec) as s:
+        s.upload(b)
+        s.step(steps)
+        f = s.download()
+    return P(f)
+
+
+@pytest.fixture(scope="module")
+def config4():
+    # the differentiation workload at a fifth of its size: 20,000 ring
+    # bodies, 200 steps, a checkpoint every 20 steps; the loss is the summed
+    # squared distance of the final positions to those of a run whose
+    # velocities carry seeded N(0, 1e-4) noise (the FP64 GPU step)
+    n, steps, dt = 20_000, 200, 1e-3
+    b = K.ring(n, seed=28)
+    rng = np.random.default_rng(29)
+    ref = dict(b)
+    for k in ("vx", "vy", "vz"):
+        ref[k] = b[k] + rng.normal(size=n) * 1e-4
+    pref = gpu_final(ref, steps, dt, K.KD_PREC_FP64)
+    # central differences (eps 1e-7) of the FP64 forward along 16 seeded
+    # directions in (velocity, mass)
+    eps, dirs = 1e-7, []
+    for _ in range(16):
+        dv = rng.normal(size=(n, 3))
+        dm = b["m"] * rng.normal(size=n)
+
+        def loss(sgn):
+            c = dict(b)
+            c["vx"], c["vy"], c["vz"] = (V(b) + sgn * eps * dv).T
+            c["m"] = b["m"] + sgn * eps * dm
+            return ((gpu_final(c, steps, dt, K.KD_PREC_FP64) - pref) ** 2).sum()
+        dirs.append((dv, dm, (loss(1) - loss(-1)) / (2 * eps)))
+    return dict(b=b, pref=pref, steps=steps, dt=dt, dirs=dirs)
+
+
+def directional_devs(c, prec):
+    _, gv, gm, pN = gpu_grads(c["b"], c["pref"], c["steps"], 20, 0.3, c["dt"], prec)
+    devs = []
+    for dv, dm, fd in c["dirs"]:
+        ad = (gv * dv).sum() + (gm * dm).sum()
+        devs.append(abs(fd - ad) / abs(ad))
+    return np.array(devs), pN
+
+
+def test_config4_fp64_central_differences(config4):
+    devs, _ = directional_devs(config4, K.KD_PREC_FP64)
+    print(f"config-4 FP64 adjoint vs FP64 central differences: median {np.median(devs):.3e} "
+          f"max {devs.max():.3e}")
+    assert devs.max() <= 1e-5, devs
+
+
+def test_config4_mixed_against_fp64_differences(config4):
+    # the mixed forward reads FP32 positions, so a 1e-7 perturbation of the
+    # velocities (1e-10 in position a step) is below its resolution and its
+    # own differences are no derivative; the mixed adjoint is checked against
+    # the FP64 differences instead.  To first order it is the FP64 gradient
+    # with the final residual r = p_N - p_ref moved by the two forwards'
+    # trajectory difference d = p_N(mixed) - p_N(FP64) (the loss gradient is
+    # 2 J^T r), plus the FP32 rounding of the terms (relative ~1e-6 at most,
+    # the force-error bound).  So each directional derivative may deviate by
+    # about |d| / |r| + 1e-6; the test allows 10x that at the median and 100x
+    # at the maximum over the 16 directions.
+    c = config4
+    devs, pmix = directional_devs(c, K.KD_PREC_MIXED)
+    p64 = gpu_final(c["b"], c["steps"], c["dt"], K.KD_PREC_FP64)
+    scale = np.linalg.norm(pmix - p64) / np.linalg.norm(p64 - c["pref"]) + 1e-6
+    print(f"config-4 mixed adjoint vs FP64 central differences: median {np.median(devs):.3e} "
+          f"max {devs.max():.3e}; |d|/|r| + 1e-6 = {scale:.3e}")
+    assert np.median(devs) <= 10 * scale, (devs, scale)
+    assert devs.max() <= 100 * scale, (devs, scale)

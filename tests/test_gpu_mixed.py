@@ -22,13 +22,15 @@ def rel(a, ref):
     return np.linalg.norm(a - ref, axis=1) / np.linalg.norm(ref, axis=1)
 
 
+@pytest.mark.parametrize("theta", [0.3, 0.5])
 @pytest.mark.parametrize("prec", [K.KD_PREC_MIXED, K.KD_PREC_MIXED_D64])
-def test_error_bounds_against_fp64_direct_sum(prec):
+def test_error_bounds_against_fp64_direct_sum(prec, theta):
     # the benchmark's error metric on a smaller ring: sampled bodies against
-    # the FP64 direct sum
+    # the FP64 direct sum (itself checked against the oracle in
+    # test_gpu_direct.py), at both opening angles of the workload
     n = 200_000
     b = K.ring(n, seed=1)
-    with K.Simulation(n, precision=prec) as s:
+    with K.Simulation(n, theta=theta, precision=prec) as s:
         s.upload(b)
         a = s.accel()
         idx = np.sort(np.random.default_rng(5).choice(n, 2048, replace=False))
@@ -39,12 +41,13 @@ def test_error_bounds_against_fp64_direct_sum(prec):
     assert np.percentile(r, 99) <= P99_TOL, np.percentile(r, 99)
 
 
+@pytest.mark.parametrize("theta", [0.3, 0.5])
 @pytest.mark.parametrize("prec", [K.KD_PREC_MIXED, K.KD_PREC_MIXED_D64])
-def test_close_to_fp64_tree_walk(prec):
+def test_close_to_fp64_tree_walk(prec, theta):
     # same tree, FP64 reference arithmetic: only precision (and the rare
     # acceptance decision at the threshold) differs
     b = K.ring(100_000, seed=2)
-    r = rel(accel(b, prec), accel(b, K.KD_PREC_FP64))
+    r = rel(accel(b, prec, theta), accel(b, K.KD_PREC_FP64, theta))
     assert np.median(r) <= MED_TOL and np.percentile(r, 99) <= P99_TOL, (
         np.median(r), np.percentile(r, 99))
 

@@ -122,3 +122,29 @@ def test_tape_replay_reproduces_run():
     again, _ = oracle.adj_forward(b, 1e-2, 6, 0.5, replay=tape)
     for k in ("x", "y", "z", "vx", "vy", "vz"):
         np.testing.assert_array_equal(fin[k], again[k])
+
+
+@pytest.mark.parametrize("case", ["ring", "clustered"])
+def test_recorded_run_equals_simple_sim(case):
+    # the recorded (frozen-topology) forward is simple_sim itself: the same
+    # trees and the same accept/open decisions, only the node moments
+    # (re-summed over the members in particle order) and the force sums
+    # (one running sum in visit order, where orc_simple_sim adds left +
+    # right as accel_recur does, array_kd_tree.rs:611-613) round in another
+    # order.  Each step's accelerations then differ by a few ulps of the
+    # summed terms: the velocity changes v - v0 agree to 1e-12 of the
+    # largest change, and each step's position update p += dt v may round
+    # the other way, so positions agree to 2 ulps of their magnitude a step.
+    if case == "ring":
+        b, dt, steps, theta = K.ring(2000, seed=41), 1e-3, 10, 0.3
+    else:
+        b, dt, steps, theta = clustered(1500, 42), 1e-2, 10, 0.5
+    fin, _ = oracle.adj_forward(b, dt, steps, theta)
+    ref = oracle.simple_sim(b, dt=dt, steps=steps, theta=theta)
+    for k in ("vx", "vy", "vz"):
+        dv = np.abs(fin[k] - ref[k]).max()
+        kick = np.abs(ref[k] - b[k]).max()
+        assert dv <= 1e-12 * kick, (k, dv / kick)
+    pmax = max(np.abs(ref[k]).max() for k in "xyz")
+    for k in ("x", "y", "z"):
+        assert np.abs(fin[k] - ref[k]).max() <= 2 * steps * 2.0 ** -52 * pmax, k
