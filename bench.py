@@ -7,8 +7,12 @@ and P2P with compensated FP32 sums; FP64 state, integrator and node moments).
 Beside it, on the same bodies: the plain FP64 GPU step, the median / p99
 relative force error of the mixed step against an FP64 direct sum on 4,096
 seeded bodies, the traversal kernel's roofline from its own interaction
-counters and HIP-event durations, and the reference's C implementation timed
-on the host cores on a bounded sample.
+counters and HIP-event durations, and the reference's own C simple_sim
+(Parallel/CVersion, OpenMP) timed over one whole step of the same bodies on
+the host cores.  Extra legs, each in its own key: the same step at
+theta = 0.5; the FP64 step at 1 M bodies over 10 steps; the 512-system
+ensemble; the differentiation workload; standalone selection over 100 M
+keys; the build alone on circular_orbits(10 M).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -32,6 +36,7 @@ METRIC = "particle-steps/s @N=10M theta=0.3: FP32-mixed vs FP64 GPU @ median for
 PEAK_FP32_VALU_TFLOPS = 157.3  # MI355X published vector FP32 peak
 PEAK_FP64_VALU_TFLOPS = 78.6   # MI355X published vector FP64 peak
 PEAK_HBM_GBS = 8000.0
+CUS, CLOCK_HZ = 256, 2.4e9     # MI355X compute units and peak engine clock
 
 
 def log(*a):
@@ -74,41 +79,28 @@ def force_error(bodies, n, theta, leaf, precision, k=4096, seed=2024):
     return float(np.median(rel)), float(np.percentile(rel, 99)), float(rel.max())
 
 
-def cpu_baseline(bodies, n, sample, threads, dt):
-    """The reference's own C implementation (Parallel/CVersion, OpenMP) timed
-    on the host: full sequential build, calc_accel over a random sample of
-    bodies in parallel, kick-drift over all bodies; scaled to one step."""
+def cpu_baseline(bodies, n, steps, threads, dt):
+    """The reference's own C simple_sim (Parallel/CVersion/src/kdtree.c:221-260:
+    build_tree on one thread, calc_accel of every body and the kick-drift as
+    OpenMP parallel-for loops), run whole for `steps` steps on a copy of the
+    bodies and timed around the call; without the reference build, the
+    oracle's single-thread port over the same steps."""
     import oracle
-    rng = np.random.default_rng(7)
-    idx = np.sort(rng.choice(n, size=min(sample, n), replace=False)).astype(np.int64)
-    R = oracle.ref_lib("par")
-    f = {k: np.ascontiguousarray(bodies[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
-    if R is not None:
-        import ctypes as C
-        d = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))  # noqa: E731
-        out = np.empty(3 * len(idx))
-        t = np.zeros(3)
-        R.ref_cpu_sample(n, d(f["x"]), d(f["y"]), d(f["z"]), d(f["vx"]), d(f["vy"]),
-                         d(f["vz"]), d(f["m"]), len(idx),
-                         idx.ctypes.data_as(C.POINTER(C.c_int64)), threads, dt, d(out), d(t))
+    r = oracle.ref_simple_sim_timed(bodies, dt=dt, steps=steps, threads=threads)
+    if r is not None:
+        _, sec = r
         kind, cores = "reference", threads
-        t_build, t_acc, t_int = t
-        what = (f"reference C (Parallel/CVersion kdtree.c, gcc -O2, OpenMP {threads} threads): "
-                f"sequential build_tree of all {n} bodies {t_build:.2f} s, calc_accel of "
-                f"{len(idx)} random bodies {t_acc:.2f} s, kick-drift of all {t_int:.3f} s")
+        what = (f"reference C simple_sim (Parallel/CVersion kdtree.c:221-260, gcc -O2, OpenMP "
+                f"{threads} threads, sequential build_tree), {steps} whole step(s) of all {n} "
+                f"bodies: {sec:.2f} s")
     else:
-        # the oracle's port (single thread): build + sampled accel
         t0 = time.perf_counter()
-        oracle.build_tree(f, leaf=7)
-        t1 = time.perf_counter()
-        oracle.accel(f, idx=idx)
-        t2 = time.perf_counter()
-        t_build, t_acc, t_int = t1 - t0, (t2 - t1) - (t1 - t0), 0.0
+        oracle.simple_sim(bodies, dt=dt, steps=steps)
+        sec = time.perf_counter() - t0
         kind, cores = "port", 1
-        what = f"oracle port, 1 thread: build {t_build:.2f} s, accel of {len(idx)} bodies"
-    step_s = t_build + t_acc * n / len(idx) + t_int
-    return dict(value=n / step_s, unit="particle-steps/s", cores=cores, kind=kind,
-                sample=what + f"; one step extrapolated = {step_s:.2f} s")
+        what = f"oracle port (1 thread), {steps} whole step(s) of all {n} bodies: {sec:.2f} s"
+    return dict(value=n * steps / sec, unit="particle-steps/s", cores=cores, kind=kind,
+                sample=what, seconds=sec, steps=steps, n=n)
 
 
 def batch_sizes(nsys, seed):
@@ -158,13 +150,15 @@ def run_batch(nsys, steps, loop_steps, theta, leaf, dt, seed):
                 unit="particle-steps/s", precision="mixed", graph=True)
 
 
-def run_adjoint(n, steps, every, theta, leaf, dt, seed, ndir, ndir64, cpu_n):
-    """Differentiation workload: gradients of sum |p_N - p_ref|^2 (p_ref from
-    velocities + N(0, 1e-4)) with respect to initial velocities and masses,
-    mixed precision, checkpoint every `every` steps; checked against central
-    differences (eps 1e-7) of GPU forward runs along seeded random
-    directions, and (at cpu_n bodies) against the oracle's FP64 reference
-    gradient."""
+def run_adjoint(n, steps, every, theta, leaf, dt, seed, ndir, cpu_n):
+    """Differentiation workload: gradients of sum |p_N - p_ref|^2 (p_ref: the
+    FP64 run from velocities + N(0, 1e-4)) with respect to initial velocities
+    and masses, checkpoint every `every` steps, in both precisions; each
+    directional derivative is compared with central differences (eps 1e-7)
+    of the FP64 forward along `ndir` seeded random directions (the mixed
+    forward reads FP32 positions, so a 1e-7 perturbation is below its
+    resolution and its own differences are no derivative), and (at cpu_n
+    bodies) the gradients with the oracle's FP64 reference gradient."""
     import torch
     from multilanguagekdtree_amd.autograd import Adjoint
     import oracle
@@ -181,7 +175,7 @@ def run_adjoint(n, steps, every, theta, leaf, dt, seed, ndir, ndir64, cpu_n):
             s.step(steps)
             return P(s.download())
 
-    def grads(b, pref, prec, timing=False):
+    def grads(b, pref, prec):
         sim = Adjoint(len(b["x"]), steps, every, theta=theta, dt=dt, leaf_size=leaf,
                       precision=prec)
         x0, v0, m = cuda(P(b)), cuda(V(b)), cuda(b["m"])
@@ -193,38 +187,45 @@ def run_adjoint(n, steps, every, theta, leaf, dt, seed, ndir, ndir64, cpu_n):
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         sim.close()
-        return gv.cpu().numpy(), gm.cpu().numpy(), t1 - t0, t2 - t1
+        return gv.cpu().numpy(), gm.cpu().numpy(), t1 - t0, t2 - t1, pN.cpu().numpy()
 
     rng = np.random.default_rng(seed + 7)
     b = K.ring(n, seed=seed + 3)
     ref = dict(b)
     for k in ("vx", "vy", "vz"):
         ref[k] = b[k] + rng.normal(size=n) * 1e-4
-    out = {"n": n, "steps": steps, "checkpoint_every": every, "precision": "mixed"}
-    for name, prec, nd in (("mixed", K.KD_PREC_MIXED, ndir), ("fp64", K.KD_PREC_FP64, ndir64)):
-        pref = final_positions(ref, prec)
-        grads(b, pref, prec)  # warm-up
-        gv, gm, tf, tb = grads(b, pref, prec)
-        devs = []
-        for _ in range(nd):
-            dv = rng.normal(size=(n, 3))
-            dm = b["m"] * rng.normal(size=n)
-            eps = 1e-7
+    pref = final_positions(ref, K.KD_PREC_FP64)
+    eps, fds = 1e-7, []
+    t0 = time.perf_counter()
+    for _ in range(ndir):
+        dv = rng.normal(size=(n, 3))
+        dm = b["m"] * rng.normal(size=n)
 
-            def loss(sgn):
-                c = dict(b)
-                c["vx"], c["vy"], c["vz"] = (V(b) + sgn * eps * dv).T
-                c["m"] = b["m"] + sgn * eps * dm
-                return ((final_positions(c, prec) - pref) ** 2).sum()
-            fd = (loss(1) - loss(-1)) / (2 * eps)
-            ad = float((gv * dv).sum() + (gm * dm).sum())
-            devs.append(abs(fd - ad) / abs(ad))
+        def loss(sgn):
+            c = dict(b)
+            c["vx"], c["vy"], c["vz"] = (V(b) + sgn * eps * dv).T
+            c["m"] = b["m"] + sgn * eps * dm
+            return ((final_positions(c, K.KD_PREC_FP64) - pref) ** 2).sum()
+        fds.append((dv, dm, (loss(1) - loss(-1)) / (2 * eps)))
+    log(f"[bench] adjoint: {ndir} FP64 central differences in {time.perf_counter() - t0:.1f} s")
+    p64 = final_positions(b, K.KD_PREC_FP64)
+    out = {"n": n, "steps": steps, "checkpoint_every": every, "fd_directions": ndir,
+           "fd_eps": eps, "fd_forward": "fp64"}
+    for name, prec in (("mixed", K.KD_PREC_MIXED), ("fp64", K.KD_PREC_FP64)):
+        grads(b, pref, prec)  # warm-up
+        gv, gm, tf, tb, pN = grads(b, pref, prec)
+        devs = [abs(fd - float((gv * dv).sum() + (gm * dm).sum())) /
+                abs(float((gv * dv).sum() + (gm * dm).sum())) for dv, dm, fd in fds]
         out[name] = {"forward_s": tf, "backward_s": tb,
                      "forward_particle_steps_per_s": n * steps / tf,
                      "fwd_bwd_particle_steps_per_s": n * steps / (tf + tb),
-                     "fd_directions": nd,
-                     "fd_rel_dev_median": float(np.median(devs)) if devs else None,
-                     "fd_rel_dev_max": float(np.max(devs)) if devs else None}
+                     "fd_rel_dev_median": float(np.median(devs)),
+                     "fd_rel_dev_max": float(np.max(devs))}
+        if name == "mixed":
+            # first-order size of the deviation: the two forwards' final
+            # positions differ by d, and the gradient is 2 J^T (p_N - p_ref)
+            out[name]["traj_shift_over_residual"] = float(
+                np.linalg.norm(pN - p64) / np.linalg.norm(p64 - pref))
         log(f"[bench] adjoint {name}: {out[name]}")
     # FP64 CPU reference gradient (the oracle, frozen topology) at cpu_n bodies
     bc = K.ring(cpu_n, seed=seed + 5)
@@ -237,7 +238,7 @@ def run_adjoint(n, steps, every, theta, leaf, dt, seed, ndir, ndir64, cpu_n):
     del tape
     cmp = {}
     for name, prec in (("mixed", K.KD_PREC_MIXED), ("fp64", K.KD_PREC_FP64)):
-        gv, gm, _, _ = grads(bc, P(prc), prec)
+        gv, gm, _, _, _ = grads(bc, P(prc), prec)
         cmp[name] = {"v_rel": float(np.linalg.norm(gv - ov) / np.linalg.norm(ov)),
                      "m_rel": float(np.linalg.norm(gm - om) / np.linalg.norm(om))}
     out["vs_cpu_fp64_gradient"] = dict(n=cpu_n, steps=steps, **cmp)
@@ -323,6 +324,42 @@ def run_select(n, seed, reps):
             "speedup_vs_cpu": t_cpu / float(np.median(times))}
 
 
+def tree_depth(n, leaf):
+    """Levels below the root of the tree of n bodies (every split at floor(len/2))."""
+    d, mx = 0, n
+    while mx > leaf:
+        mx = (mx + 1) // 2
+        d += 1
+    return d
+
+
+def compulsory_bytes(n, leaf):
+    """Compulsory HBM bytes per particle-step: read p, v, m and write p, v in
+    FP64 (104 B), 16 B per build level (8 B key + 4 B index in + 4 B out) and
+    ~20 B of node records."""
+    return 104 + 16 * tree_depth(n, leaf) + 20
+
+
+def step_leg(bodies, n, steps, warmup, theta, leaf, dt, precs=("fp64", "mixed"), err=True):
+    """One workload timed in the given precisions (+ the mixed force error)."""
+    out = {"n": n, "theta": theta, "leaf_size": leaf, "steps": steps}
+    for name in precs:
+        prec = K.KD_PREC_MIXED if name == "mixed" else K.KD_PREC_FP64
+        r = run_gpu(bodies, n, prec, steps, warmup, theta, leaf, dt)
+        out[name] = {"value": r["rate"], "ms_per_step": r["ms_per_step"],
+                     "phases_ms": {k: round(r[k], 3)
+                                   for k in ("build_ms", "traverse_ms", "kick_drift_ms")},
+                     "algorithmic_hbm_gbps": round(compulsory_bytes(n, leaf) * r["rate"] / 1e9, 1)}
+        log(f"[bench] {name} n={n} theta={theta}: {json.dumps(out[name])}")
+    if "fp64" in out and "mixed" in out:
+        out["speedup_mixed_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    if err and "mixed" in out:
+        med, p99, mx = force_error(bodies, n, theta, leaf, K.KD_PREC_MIXED)
+        out["mixed"]["force_error"] = dict(median=med, p99=p99, max=mx, samples=4096)
+        log(f"[bench] force error n={n} theta={theta}: {out['mixed']['force_error']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -332,12 +369,16 @@ def main():
     ap.add_argument("--theta", type=float, default=0.3)
     ap.add_argument("--leaf", type=int, default=7)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=65536)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    # whole steps of the reference C simple_sim on the headline bodies (one
+    # 10 M step takes about a minute on 16 host cores) and at 1 M bodies
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-config1-steps", type=int, default=3)
+    ap.add_argument("--cpu-repeats", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
     ap.add_argument("--only", choices=["mixed", "fp64", "both", "batch", "adjoint", "copy",
-                                       "build", "select"], default="both")
+                                       "build", "select", "cpu"], default="both")
     ap.add_argument("--select-n", type=int, default=100_000_000)
     ap.add_argument("--copy-mib", type=int, default=1024)
     # MAX_PARTS of the crate the build-only benchmark comes from
@@ -347,12 +388,16 @@ def main():
     ap.add_argument("--adj-steps", type=int, default=200)
     ap.add_argument("--adj-every", type=int, default=20)
     ap.add_argument("--adj-dirs", type=int, default=64)
-    ap.add_argument("--adj-dirs-fp64", type=int, default=64)
     ap.add_argument("--adj-cpu-n", type=int, default=2000)
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--batch-steps", type=int, default=20)
-    ap.add_argument("--batch-loop-steps", type=int, default=2)
+    ap.add_argument("--batch-loop-steps", type=int, default=5)
+    ap.add_argument("--config1-n", type=int, default=1_000_000)
+    ap.add_argument("--config1-steps", type=int, default=10)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="headline legs only: no theta 0.5, 1 M, adjoint, selection or "
+                         "build-only legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -360,6 +405,7 @@ def main():
         sys.exit(2)
 
     n, dt = args.n, 1e-3
+    threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", 0)) or (os.cpu_count() or 1)
     if args.only == "select":
         print(json.dumps(run_select(args.select_n, args.seed, 3)), flush=True)
         return
@@ -383,7 +429,7 @@ def main():
     if args.only == "adjoint":
         log(f"[bench] adjoint: n={args.adj_n} steps={args.adj_steps}")
         ra = run_adjoint(args.adj_n, args.adj_steps, args.adj_every, args.theta, args.leaf, dt,
-                         args.seed, args.adj_dirs, args.adj_dirs_fp64, args.adj_cpu_n)
+                         args.seed, args.adj_dirs, args.adj_cpu_n)
         print(json.dumps(ra), flush=True)
         return
     if args.only == "batch":
@@ -391,6 +437,18 @@ def main():
         rb = run_batch(args.batch_systems, args.batch_steps, args.batch_loop_steps, args.theta,
                        args.leaf, dt, args.seed)
         print(json.dumps(rb), flush=True)
+        return
+    if args.only == "cpu":
+        # the reference C simple_sim alone: median of --cpu-repeats runs
+        out = {}
+        for nn, st in ((n, args.cpu_steps), (args.config1_n, args.config1_steps)):
+            b = K.ring(nn, seed=args.seed)
+            runs = [cpu_baseline(b, nn, st, threads, dt) for _ in range(args.cpu_repeats)]
+            vals = sorted(r["value"] for r in runs)
+            out[f"n{nn}_steps{st}"] = dict(runs[0], value=vals[len(vals) // 2],
+                                          values=vals, repeats=args.cpu_repeats)
+            log(f"[bench] cpu n={nn} steps={st}: {out[f'n{nn}_steps{st}']}")
+        print(json.dumps(out), flush=True)
         return
     log(f"[bench] generating ring ICs n={n} seed={args.seed}")
     bodies = K.ring(n, seed=args.seed)
@@ -429,7 +487,9 @@ def main():
         except Exception:
             traffic = None
     # vector-ALU counters of the walk from the committed counter pass
-    # (profiles/valu_ring10M_mixed.md): VALUBusy and VALUUtilization
+    # (profiles/valu_ring10M_mixed.md): VALU instructions per wavefront and
+    # VALUUtilization (the share of lanes doing work in issued instructions);
+    # the issue rate is per CU and clock over this run's walk duration
     valu = None
     vf = ROOT / "profiles" / "valu.json"
     if vf.exists() and main_name == "mixed":
@@ -437,24 +497,49 @@ def main():
             valu = json.loads(vf.read_text()).get(f"traverse_{main_name}_n{n}")
         except Exception:
             valu = None
+    issue = None
+    if valu is not None:
+        waves = (n + 63) // 64
+        issue = valu["valu_insts_per_wave"] * waves / (r["traverse_ms"] * 1e-3 * CUS * CLOCK_HZ)
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
                     frac=round(achieved / peak, 4), traffic=traffic,
                     kernel=f"k_traverse_{'mixed' if main_name == 'mixed' else 'f64'}",
                     kernel_ms=round(r["traverse_ms"], 3),
                     hbm_gbps=(None if traffic is None
                               else round(traffic / (r["traverse_ms"] * 1e-3) / 1e9, 1)),
-                    valu_busy_pct=None if valu is None else round(valu["valu_busy_pct"], 1),
+                    valu_insts_per_wave=None if valu is None else round(valu["valu_insts_per_wave"]),
+                    valu_issue_per_cu_cycle=None if issue is None else round(issue, 3),
                     valu_utilization_pct=(None if valu is None
                                           else round(valu["valu_utilization_pct"], 1)),
                     flops_per_launch=flops,
                     per_particle=dict(tests=r["tests"] / n, m2p=r["m2p"] / n, p2p=r["p2p"] / n))
 
+    extra = {}
+    if not args.no_extra:
+        log("[bench] theta 0.5 leg")
+        extra["theta_0.5"] = step_leg(bodies, n, args.steps, args.warmup, 0.5, args.leaf, dt)
+    del bodies
+
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or (os.cpu_count() or 1)
-        log(f"[bench] CPU baseline: reference C, {threads} threads, sample {args.cpu_sample}")
-        cpu = cpu_baseline(bodies, n, args.cpu_sample, threads, dt)
+        log(f"[bench] CPU baseline: reference C simple_sim, {threads} threads, "
+            f"{args.cpu_steps} whole step(s) at n={n}")
+        cpu = cpu_baseline(K.ring(n, seed=args.seed), n, args.cpu_steps, threads, dt)
         log(f"[bench] cpu {cpu}")
+
+    if not args.no_extra:
+        n1 = args.config1_n
+        log(f"[bench] FP64 leg n={n1}, {args.config1_steps} steps")
+        b1 = K.ring(n1, seed=args.seed)
+        extra["config1_fp64"] = step_leg(b1, n1, args.config1_steps, args.warmup, args.theta,
+                                         args.leaf, dt, precs=("fp64",), err=False)
+        if not args.no_cpu_baseline and args.cpu_config1_steps > 0:
+            c1 = cpu_baseline(b1, n1, args.cpu_config1_steps, threads, dt)
+            extra["config1_fp64"]["cpu_reference"] = c1
+            extra["config1_fp64"]["speedup_vs_cpu"] = (
+                extra["config1_fp64"]["fp64"]["value"] / c1["value"])
+            log(f"[bench] cpu n={n1}: {c1}")
+        del b1
 
     batch = None
     if not args.no_batch:
@@ -462,6 +547,17 @@ def main():
         batch = run_batch(args.batch_systems, args.batch_steps, args.batch_loop_steps,
                           args.theta, args.leaf, dt, args.seed)
         log(f"[bench] batch {json.dumps(batch)}")
+
+    if not args.no_extra:
+        log(f"[bench] adjoint: n={args.adj_n} steps={args.adj_steps}")
+        extra["adjoint"] = run_adjoint(args.adj_n, args.adj_steps, args.adj_every, args.theta,
+                                       args.leaf, dt, args.seed, args.adj_dirs, args.adj_cpu_n)
+        log("[bench] selection leg")
+        extra["select"] = run_select(args.select_n, args.seed, 3)
+        log(f"[bench] select {extra['select']}")
+        log("[bench] build-only leg")
+        extra["build_only"] = run_build_only(n, args.build_leaf, args.seed, 5)
+        log(f"[bench] build only {extra['build_only']}")
 
     out = {
         "metric": (METRIC if (n, args.theta) == (10_000_000, 0.3) else
@@ -485,6 +581,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "phases_ms": {k: round(r[k], 3) for k in ("build_ms", "traverse_ms", "kick_drift_ms")},
+        "algorithmic_hbm_gbps": round(compulsory_bytes(n, args.leaf) * r["rate"] / 1e9, 1),
         "force_error": err,
         "batch": batch,
     }
@@ -493,6 +590,9 @@ def main():
                            "phases_ms": {k: round(res["fp64"][k], 3)
                                          for k in ("build_ms", "traverse_ms", "kick_drift_ms")}}
         out["speedup_mixed_vs_fp64"] = res["mixed"]["rate"] / res["fp64"]["rate"]
+    if cpu is not None:
+        out["speedup_vs_cpu"] = r["rate"] / cpu["value"]
+    out.update(extra)
     print(json.dumps(out), flush=True)
 
 

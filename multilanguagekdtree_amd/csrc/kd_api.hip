@@ -233,11 +233,10 @@ int kd_destroy(kd_ctx *c) {
 }
 
 // Buffer sizes beyond the body capacity: sort scratch, per-tile level
-// counts, tie flags, and the number of systems.
+// counts, and the number of systems.
 struct Sizing {
   size_t sort_bytes;
   size_t cnt_entries;
-  size_t tieflag_entries;
   int nsys;
 };
 
@@ -293,7 +292,6 @@ static int create_impl(kd_ctx *c, int64_t cap, const Sizing &z) {
   HIP_TRY(hipMemsetAsync(c->pos64, 0, sizeof(double4) * (N + BODY_PAD), c->stream));
   HIP_TRY(hipMemsetAsync(c->pos32, 0, sizeof(float4) * (N + BODY_PAD), c->stream));
   ALLOC(c->allocs, c->b.cnt, z.cnt_entries);
-  ALLOC(c->allocs, c->b.tieflag, z.tieflag_entries);
   ALLOC(c->allocs, c->dsys, z.nsys);
   ALLOC(c->allocs, c->sob, N / 64 + 1);
   ALLOC(c->allocs, c->counters, 8);
@@ -326,7 +324,7 @@ int kd_create(int64_t cap_n, const kd_params *params, kd_ctx **out) {
   // cap/LVL_TILE + 2^level tiles
   const size_t max_seg = 2 * ((size_t)cap_n / SUB_MAX) + 64;
   Sizing z{sort_temp_bytes((uint32_t)cap_n, 1),
-           3 * ((size_t)cap_n / LVL_TILE + max_seg + 64), max_seg, 1};
+           3 * ((size_t)cap_n / LVL_TILE + max_seg + 64), 1};
   rc = create_impl(c, cap_n, z);
   if (rc) {
     std::string msg = g_err;
@@ -1156,8 +1154,7 @@ extern "C" int kd_batch_create(int64_t nsys, const int64_t *n, const kd_params *
     f.max_sub_level = std::max(f.max_sub_level, (int)d.sub_level);
     f.max_n = std::max(f.max_n, d.n);
   }
-  Sizing z{sort_temp_bytes((uint32_t)body, (int)nsys), level_cnt_entries(f), tieflag_entries(f),
-           (int)nsys};
+  Sizing z{sort_temp_bytes((uint32_t)body, (int)nsys), level_cnt_entries(f), (int)nsys};
   auto bail = [&](int code) {
     std::string msg = g_err;
     kd_batch_destroy(b);

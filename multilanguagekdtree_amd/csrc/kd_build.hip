@@ -21,6 +21,7 @@
 // Ties on a split coordinate are resolved by original index, lazily: only a
 // run of equal keys that straddles the median is reordered.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 
@@ -227,9 +228,11 @@ __global__ __launch_bounds__(256) void k_range(const double *__restrict__ x,
 // sort moves them behind every system's bodies).
 constexpr uint32_t PAD_KEY = 0xFFFFFFFFu;
 
+// The sort key of every body position j (the values the sort carries are
+// the positions themselves, read from a counting iterator).
 __global__ void k_make_keys32(const double *__restrict__ p, const unsigned long long *range,
-                              uint32_t *__restrict__ keys, uint32_t *__restrict__ ids, uint32_t n,
-                              uint32_t *runs, const uint32_t *__restrict__ sob,
+                              uint32_t *__restrict__ keys, uint32_t n, uint32_t *runs,
+                              const uint32_t *__restrict__ sob,
                               const SysDesc *__restrict__ sys) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j == 0) runs[0] = 0;
@@ -243,7 +246,6 @@ __global__ void k_make_keys32(const double *__restrict__ p, const unsigned long 
     if (j - S.body_off >= S.n) k = PAD_KEY;
   }
   keys[j] = k;
-  ids[j] = j;
 }
 
 // Runs of equal keys: a run of at most FIX_SHORT ids is put in (coordinate,
@@ -476,64 +478,17 @@ struct ListBufs {
   __device__ uint32_t *out(uint32_t m, int d) const { return ((m >> d) & 1u) ? A.l[d] : B.l[d]; }
 };
 
-__global__ void k_level_split(ListBufs LB, Coords P, TreeDev t0, const SysDesc *sys,
-                              uint32_t level, uint32_t *tieflag, uint32_t tf_stride) {
-  const SysDesc S = sys[blockIdx.y];
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t nseg = 1u << level;
-  if (k >= nseg || level >= S.sub_level) return;
-  const TreeDev t = tree_at(t0, S.node_off);
-  tieflag += blockIdx.y * tf_stride;
-  uint32_t h = nseg - 1 + k;
-  uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
-  const uint32_t m = t.lmask[h];
-  double ext[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const uint32_t *ld = LB.in(m, d);
-    double lo = P.c[d][ld[s]];
-    double hi = P.c[d][ld[s + c - 1]];
-    ext[d] = hi - lo;
-  }
-  int sd = 0;
-  if (ext[1] > ext[sd]) sd = 1;
-  if (ext[2] > ext[sd]) sd = 2;
-  uint32_t mid = s + c / 2;
-  double kv = P.c[sd][LB.in(m, sd)[mid]];
-  double kp = P.c[sd][LB.in(m, sd)[mid - 1]];
-  const uint8_t cm = (uint8_t)(m ^ (7u ^ (1u << sd)));
-  t.lmask[2 * h + 1] = cm;
-  t.lmask[2 * h + 2] = cm;
-  t.split_dim[h] = (uint8_t)sd;
-  t.split_val[h] = kv;
-  t.size[h] = ext[sd];
-  tieflag[k] = (kp == kv) ? 1u : 0u;
-}
-
-// A run of equal keys straddles the median: the run's members that go left
-// are those with the smallest original indices.  One 1024-thread block per
-// flagged segment: radix-select the threshold index, then stably partition
-// the run (scratch = the same list's output buffer, overwritten later).
-__global__ __launch_bounds__(1024) void k_level_tiefix(ListBufs LB,
-                                                       Coords P, TreeDev t0,
-                                                       const SysDesc *sys,
-                                                       const uint32_t *oid,
-                                                       uint32_t level,
-                                                       const uint32_t *tieflag,
-                                                       uint32_t tf_stride) {
-  constexpr int NT = 1024;
-  const SysDesc S = sys[blockIdx.y];
-  uint32_t k = blockIdx.x;
-  if (level >= S.sub_level || !tieflag[blockIdx.y * tf_stride + k]) return;
-  const TreeDev t = tree_at(t0, S.node_off);
-  uint32_t h = (1u << level) - 1 + k;
-  uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
-  int sd = t.split_dim[h];
-  uint32_t *ls = LB.in(t.lmask[h], sd);
-  uint32_t *tmp = LB.out(t.lmask[h], sd);  // not live for this node: scratch
-  const double *key = P.c[sd];
-  uint32_t mid = s + c / 2;
-  double kv = key[ls[mid]];
+// A run of equal keys straddles the median of segment h (list ls of its
+// split dimension, [s, s + c)): the run's members that go left are those
+// with the smallest original indices.  The block radix-selects the threshold
+// index (the original index of the first body that goes right)
+// and stably partitions the run (scratch = the same list's output buffer,
+// not live for this node).
+template <int NT>
+__device__ void tie_fix(uint32_t *ls, uint32_t *tmp, const double *key,
+                        const uint32_t *__restrict__ oid, uint32_t s, uint32_t c) {
+  const uint32_t mid = s + c / 2;
+  const double kv = key[ls[mid]];
   __shared__ uint32_t r0s, r1s, need_s, prefix_s, mask_s;
   __shared__ uint32_t hist[256];
   __shared__ uint32_t scan_sh[NT / 64];
@@ -580,13 +535,13 @@ __global__ __launch_bounds__(1024) void k_level_tiefix(ListBufs LB,
     }
     __syncthreads();
   }
-  const uint32_t thr = prefix_s;  // original index of the first body that goes right
+  const uint32_t thr_oid = prefix_s;  // original index of the first body that goes right
   uint32_t nleft = mid - r0;
   uint32_t off_l = 0, off_r = nleft;
   for (uint32_t base = r0; base < r1; base += NT) {
     uint32_t p = base + threadIdx.x;
     uint32_t id = p < r1 ? ls[p] : 0;
-    uint32_t f = (p < r1 && oid[id] < thr) ? 1u : 0u;
+    uint32_t f = (p < r1 && oid[id] < thr_oid) ? 1u : 0u;
     uint32_t tot;
     uint32_t ex = block_excl_sum<NT>(f, scan_sh, tot);
     if (p < r1) {
@@ -600,6 +555,53 @@ __global__ __launch_bounds__(1024) void k_level_tiefix(ListBufs LB,
   }
   __syncthreads();
   for (uint32_t p = r0 + threadIdx.x; p < r1; p += NT) ls[p] = tmp[p];
+}
+
+// The split of every segment of a device-wide level, one block per segment:
+// bbox from the ends of the three sorted lists, split dimension with the
+// reference's strict '>' (array_kd_tree.rs:91-96), split value = key at mid
+// (:125) and the children's list-buffer masks; if equal keys straddle the
+// median, the block orders them (tie_fix).
+constexpr int SPLIT_THREADS = 256;
+__global__ __launch_bounds__(SPLIT_THREADS) void k_level_split(ListBufs LB, Coords P,
+                                                               TreeDev t0, const SysDesc *sys,
+                                                               const uint32_t *oid,
+                                                               uint32_t level) {
+  const SysDesc S = sys[blockIdx.y];
+  if (level >= S.sub_level) return;
+  const uint32_t k = blockIdx.x;
+  const TreeDev t = tree_at(t0, S.node_off);
+  const uint32_t h = (1u << level) - 1 + k;
+  const uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
+  const uint32_t m = t.lmask[h];
+  __shared__ int sd_s, tie_s;
+  if (threadIdx.x == 0) {
+    double ext[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const uint32_t *ld = LB.in(m, d);
+      double lo = P.c[d][ld[s]];
+      double hi = P.c[d][ld[s + c - 1]];
+      ext[d] = hi - lo;
+    }
+    int sd = 0;
+    if (ext[1] > ext[sd]) sd = 1;
+    if (ext[2] > ext[sd]) sd = 2;
+    const uint32_t mid = s + c / 2;
+    const double kv = P.c[sd][LB.in(m, sd)[mid]];
+    const double kp = P.c[sd][LB.in(m, sd)[mid - 1]];
+    const uint8_t cm = (uint8_t)(m ^ (7u ^ (1u << sd)));
+    t.lmask[2 * h + 1] = cm;
+    t.lmask[2 * h + 2] = cm;
+    t.split_dim[h] = (uint8_t)sd;
+    t.split_val[h] = kv;
+    t.size[h] = ext[sd];
+    sd_s = sd;
+    tie_s = kp == kv;
+  }
+  __syncthreads();
+  if (tie_s)
+    tie_fix<SPLIT_THREADS>(LB.in(m, sd_s), LB.out(m, sd_s), P.c[sd_s], oid, s, c);
 }
 
 __global__ __launch_bounds__(LVL_THREADS) void k_level_flags(ListBufs LB, TreeDev t0,
@@ -672,27 +674,36 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_count(ListBufs LB, TreeDe
   int sd = t.split_dim[h];
   uint32_t e = s + c;
   uint32_t base = s + q * LVL_TILE;
+  if (c == 0) return;
   for (int d = 0; d < 3; ++d) {
+    if (d == sd) continue;  // the split list is not partitioned: no count (uniform)
+    // list entries, then their flag words, each as a batch of independent
+    // loads (indices clamped into the segment); items past e count 0
+    const uint32_t *ld = LB.in(t.lmask[h], d);
+    uint32_t id[LVL_ITEMS], w[LVL_ITEMS];
+#pragma unroll
+    for (int i = 0; i < LVL_ITEMS; ++i)
+      id[i] = ld[min(base + i * LVL_THREADS + threadIdx.x, e - 1)];
+#pragma unroll
+    for (int i = 0; i < LVL_ITEMS; ++i) w[i] = bits[id[i] >> 5];
     uint32_t local = 0;
-    if (d != sd && c != 0) {
-      // list entries, then their flag words, each as a batch of independent
-      // loads (indices clamped into the segment); items past e count 0
-      const uint32_t *ld = LB.in(t.lmask[h], d);
-      uint32_t id[LVL_ITEMS], w[LVL_ITEMS];
 #pragma unroll
-      for (int i = 0; i < LVL_ITEMS; ++i)
-        id[i] = ld[min(base + i * LVL_THREADS + threadIdx.x, e - 1)];
-#pragma unroll
-      for (int i = 0; i < LVL_ITEMS; ++i) w[i] = bits[id[i] >> 5];
-#pragma unroll
-      for (int i = 0; i < LVL_ITEMS; ++i) {
-        uint32_t j = base + i * LVL_THREADS + threadIdx.x;
-        local += j < e ? (w[i] >> (id[i] & 31u)) & 1u : 0u;
-      }
+    for (int i = 0; i < LVL_ITEMS; ++i) {
+      uint32_t j = base + i * LVL_THREADS + threadIdx.x;
+      local += j < e ? (w[i] >> (id[i] & 31u)) & 1u : 0u;
     }
-    uint32_t tot;
-    block_excl_sum<LVL_THREADS>(local, sh, tot);
-    if (threadIdx.x == 0) cnt[((size_t)d * nseg + k) * T + q] = tot;
+    // block total: wavefront sums, then the first thread adds them
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) local += __shfl_xor(local, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t tot = 0;
+#pragma unroll
+      for (int w2 = 0; w2 < LVL_THREADS / 64; ++w2) tot += sh[w2];
+      cnt[((size_t)d * nseg + k) * T + q] = tot;
+    }
+    __syncthreads();
   }
 }
 
@@ -869,14 +880,39 @@ __device__ void write_internal_rec(const TreeDev &t, uint32_t h, uint32_t start,
   if (t.recs & REC_D) t.recd[h] = dd;
 }
 
+// Body range [s, s + cnt) of node c at depth lam below a segment of n0
+// bodies (every split at floor(count / 2)); false if an ancestor is a leaf,
+// so that the node does not exist.
+__device__ __forceinline__ bool sub_node(uint32_t c, int lam, uint32_t n0, int leaf,
+                                         uint32_t &s, uint32_t &cnt) {
+  s = 0;
+  cnt = n0;
+  for (int b = lam - 1; b >= 0; --b) {
+    if (cnt <= (uint32_t)leaf) return false;
+    const uint32_t half = cnt >> 1;
+    if ((c >> b) & 1u) {
+      s += half;
+      cnt -= half;
+    } else {
+      cnt = half;
+    }
+  }
+  return true;
+}
+
 __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
 #pragma clang fp contract(off)
   constexpr int NT = SUB_THREADS, IT = SUB_ITEMS;
-  __shared__ uint32_t ids[SUB_MAX];
-  __shared__ uint16_t lst[2][3][SUB_MAX];
-  __shared__ uint8_t flag[SUB_MAX];
-  __shared__ uint8_t sdarr[SUB_MAX];
+  // one LDS buffer: ids (working id of every local index), lst (the three
+  // sorted lists of local indices, double-buffered), flag, sdarr; once the
+  // bodies are placed, the node moments reuse it as staging and node sums
+  __shared__ __align__(16) unsigned char smem[18 * SUB_MAX];
+  uint32_t *ids = reinterpret_cast<uint32_t *>(smem);
+  uint16_t(*lst)[3][SUB_MAX] = reinterpret_cast<uint16_t(*)[3][SUB_MAX]>(smem + 4 * SUB_MAX);
+  uint8_t *flag = smem + 16 * SUB_MAX;
+  uint8_t *sdarr = smem + 17 * SUB_MAX;
   __shared__ uint64_t shv[NT / 64];
+
   __shared__ uint32_t shh[NT / 64];
   // per level (by parity): some segment of the level splits, so bodies are
   // still alive below it
@@ -930,22 +966,8 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
     const uint32_t hbase = (1u << (level + lam)) - 1 + (k << lam);
     // one thread per segment of this level: split decision or leaf mark
     for (uint32_t c = tid; c < (1u << lam); c += NT) {
-      uint32_t sgs = 0, sgc = n0;
-      bool ok = true;
-      for (int bb = lam - 1; bb >= 0; --bb) {
-        if (sgc <= (uint32_t)leaf) {
-          ok = false;  // an ancestor is a leaf: no such node
-          break;
-        }
-        const uint32_t half = sgc >> 1;
-        if ((c >> bb) & 1u) {
-          sgs += half;
-          sgc -= half;
-        } else {
-          sgc = half;
-        }
-      }
-      if (!ok) continue;
+      uint32_t sgs, sgc;
+      if (!sub_node(c, lam, n0, leaf, sgs, sgc)) continue;  // an ancestor is a leaf
       const uint32_t h = hbase + c;
       if (sgc <= (uint32_t)leaf) {
         sdarr[sgs] = 0xFF;  // leaf: its bodies are finished at this level
@@ -1129,12 +1151,17 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   }
   __threadfence_block();
   __syncthreads();
-  // Leaf moments, summed in canonical order (tree positions st .. st+ct-1)
-  // from the tree-ordered state: one quantity per pass (m, m x, m y, m z) is
-  // staged in LDS by all threads (the list buffers are free now), then each
-  // leaf's first thread adds its members' entries in order.
-  double *stg = reinterpret_cast<double *>(&lst[0][0][0]);
-  static_assert(sizeof(lst) >= SUB_MAX * sizeof(double), "staging needs the list buffers");
+  // Node moments, one quantity per pass (m, m x, m y, m z).  The members'
+  // terms are staged in LDS in tree order; each leaf's first thread adds its
+  // members in canonical order, and the internal nodes are summed bottom-up
+  // (left + right) in LDS, one level per barrier, so no level waits on
+  // global memory.  Every sum also goes to the node arrays.  The staged terms
+  // take SUB_MAX doubles and the node sums one per heap slot of the subtree
+  // (the deepest leaf of at most SUB_MAX bodies lies above depth
+  // log2(SUB_MAX), so fewer than SUB_MAX slots).
+  double *stg = reinterpret_cast<double *>(smem);
+  double *nsum = stg + SUB_MAX;
+  static_assert(sizeof(smem) >= 2 * SUB_MAX * sizeof(double), "moments need 16 SUB_MAX bytes");
   for (int q = 0; q < 4; ++q) {
     const double *pc = q == 1 ? a.out.x : (q == 2 ? a.out.y : a.out.z);
     for (uint32_t i = tid; i < n0; i += NT) {
@@ -1150,47 +1177,38 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
       double sum = 0.0;
       for (uint32_t r = 0; r < ct[it]; ++r) sum += stg[st[it] + r];
       dst[hl[it]] = sum;
+      // the leaf's slot in the subtree's own heap: hl + 1 = 2^(level + lam)
+      // + (k << lam) + index within depth lam
+      const uint32_t lam = (31 - __clz(hl[it] + 1)) - level;
+      nsum[hl[it] + (1u << lam) - (1u << (level + lam)) - (k << lam)] = sum;
     }
     __syncthreads();
+    for (int lam = lam_end - 1; lam >= 0; --lam) {
+      const uint32_t hbase = (1u << (level + lam)) - 1 + (k << lam);
+      for (uint32_t c = tid; c < (1u << lam); c += NT) {
+        uint32_t s, cnt;
+        if (!sub_node(c, lam, n0, leaf, s, cnt) || cnt <= (uint32_t)leaf) continue;
+        const uint32_t rel = (1u << lam) - 1 + c;
+        const double v = nsum[2 * rel + 1] + nsum[2 * rel + 2];
+        nsum[rel] = v;
+        dst[hbase + c] = v;
+      }
+      __syncthreads();
+    }
   }
 #pragma unroll
   for (int it = 0; it < IT; ++it) {
     const uint32_t j = tid * IT + it;
     if (j < n0 && j == st[it]) write_leaf_rec(t, hl[it], s0l + st[it], ct[it]);
   }
-
-  // moments bottom-up over this subtree's internal nodes
-  __threadfence_block();
-  __syncthreads();
-  for (int lam = lam_end - 1; lam >= 0; --lam) {
-    const uint32_t hbase = (1u << (level + lam)) - 1 + (k << lam);
-    for (uint32_t c = tid; c < (1u << lam); c += NT) {
-      uint32_t s = 0, cnt = n0;
-      bool ok = true;
-      for (int b = lam - 1; b >= 0; --b) {
-        if (cnt <= (uint32_t)leaf) {
-          ok = false;
-          break;
-        }
-        uint32_t half = cnt >> 1;
-        if ((c >> b) & 1u) {
-          s += half;
-          cnt -= half;
-        } else {
-          cnt = half;
-        }
-      }
-      if (!ok || cnt <= (uint32_t)leaf) continue;
-      uint32_t h = hbase + c;
-      uint32_t l = 2 * h + 1, r = 2 * h + 2;
-      t.mm[h] = t.mm[l] + t.mm[r];
-      t.mx[h] = t.mx[l] + t.mx[r];
-      t.my[h] = t.my[l] + t.my[r];
-      t.mz[h] = t.mz[l] + t.mz[r];
-      write_internal_rec(t, h, s0l + s, cnt, a.theta2);
-    }
-    __threadfence_block();
-    __syncthreads();
+  // internal records, over the subtree's heap slots (at most two a thread):
+  // their sums were written by the passes above
+  for (uint32_t rel = tid; rel + 1 < (1u << lam_end); rel += NT) {
+    const int lam = 31 - __clz(rel + 1);
+    const uint32_t c = rel + 1 - (1u << lam);
+    uint32_t s, cnt;
+    if (!sub_node(c, lam, n0, leaf, s, cnt) || cnt <= (uint32_t)leaf) continue;
+    write_internal_rec(t, (1u << (level + lam)) - 1 + (k << lam) + c, s0l + s, cnt, a.theta2);
   }
 }
 
@@ -1275,7 +1293,8 @@ static int bits_for(uint32_t v) {
 size_t sort_temp_bytes(uint32_t n, int nsys) {
   size_t bytes = 0;
   (void)rocprim::radix_sort_pairs<SortCfg>(nullptr, bytes, (uint32_t *)nullptr,
-                                           (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                           (uint32_t *)nullptr,
+                                           rocprim::counting_iterator<uint32_t>(0u),
                                            (uint32_t *)nullptr, n, 0, 32);
   if (nsys > 1) {
     size_t b2 = 0;
@@ -1301,7 +1320,6 @@ size_t level_cnt_entries(const Forest &f) {
   return m;
 }
 
-size_t tieflag_entries(const Forest &f) { return (size_t)f.nsys << f.max_sub_level; }
 
 hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t s) {
   const unsigned blocks = (f.max_slots + 255) / 256;
@@ -1337,12 +1355,13 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
     uint32_t *key32 = b.keys_in[d];
     uint32_t *key32_out = b.keys_out[d];
     k_make_keys32<<<(nsort + 255) / 256, 256, 0, sd>>>(
-        coord[d], b.range + 2 * d, key32, b.ids_in[d], nsort, b.runs[d],
+        coord[d], b.range + 2 * d, key32, nsort, b.runs[d],
         f.nsys > 1 ? f.sys_of_block : nullptr, f.sys);
     size_t bytes = b.sort_tmp_bytes;
     uint32_t *by_key = f.nsys == 1 ? b.L[d] : b.Lb[d];
-    e = rocprim::radix_sort_pairs<SortCfg>(b.sort_tmp[d], bytes, key32, key32_out, b.ids_in[d],
-                                           by_key, nsort, 0, 32, sd);
+    e = rocprim::radix_sort_pairs<SortCfg>(b.sort_tmp[d], bytes, key32, key32_out,
+                                           rocprim::counting_iterator<uint32_t>(0u), by_key,
+                                           nsort, 0, 32, sd);
     if (e != hipSuccess) return e;
     k_fix_runs<<<(nsort + 256 * FIX_PER - 1) / (256 * FIX_PER), 256, 0, sd>>>(
         key32_out, by_key, coord[d], nsort, b.range + 2 * d, b.runs[d]);
@@ -1367,14 +1386,10 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   LB.A = Lists{{b.L[0], b.L[1], b.L[2]}};
   LB.B = Lists{{b.Lb[0], b.Lb[1], b.Lb[2]}};
   Coords P{{in.x, in.y, in.z}};
-  const uint32_t tf_stride = 1u << f.max_sub_level;
   for (int lev = 0; lev < f.max_sub_level; ++lev) {
     const uint32_t nseg = 1u << lev;
     const uint32_t T = level_tiles(f, lev);
-    k_level_split<<<dim3((nseg + 255) / 256, ny), 256, 0, s>>>(LB, P, t, f.sys, lev, b.tieflag,
-                                                                tf_stride);
-    k_level_tiefix<<<dim3(nseg, ny), 1024, 0, s>>>(LB, P, t, f.sys, in.oid, lev, b.tieflag,
-                                                   tf_stride);
+    k_level_split<<<dim3(nseg, ny), SPLIT_THREADS, 0, s>>>(LB, P, t, f.sys, in.oid, lev);
     k_level_flags<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.flags);
     k_level_pack<<<(b.flag_words + 255) / 256, 256, 0, s>>>(b.flags, b.bits, b.flag_words);
     k_level_count<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.bits,

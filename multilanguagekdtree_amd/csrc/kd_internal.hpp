@@ -13,8 +13,14 @@ namespace kd {
 // 2048 (37 KB of LDS a workgroup): against 4096 (74 KB), one more device-wide
 // level but more, smaller workgroups; at 10 M bodies a build takes 4.36 ms
 // against 4.40, and a 512-system batch step 29.9 ms against 32.0.
-constexpr int SUB_THREADS = 512;
-constexpr int SUB_ITEMS = 4;
+#ifndef KD_SUB_THREADS
+#define KD_SUB_THREADS 512
+#endif
+#ifndef KD_SUB_ITEMS
+#define KD_SUB_ITEMS 4
+#endif
+constexpr int SUB_THREADS = KD_SUB_THREADS;
+constexpr int SUB_ITEMS = KD_SUB_ITEMS;
 constexpr int SUB_MAX = SUB_THREADS * SUB_ITEMS;  // 2048
 
 // Positions per block in the device-wide level kernels.  4 per thread: at
@@ -209,7 +215,7 @@ struct BuildBufs {
   // per dimension, so that the three coordinate sorts run concurrently:
   // dimension 0 on the build stream, 1 and 2 on the side streams
   uint32_t *keys_in[3], *keys_out[3];
-  uint32_t *ids_in[3];
+  uint32_t *ids_in[3];  // scratch of the long-run fix-up (and a forest's system keys)
   uint32_t *runs[3];  // long runs of equal sort keys: count, then (start, length)
   void *sort_tmp[3];
   size_t sort_tmp_bytes;  // of each sort_tmp[d]
@@ -221,16 +227,14 @@ struct BuildBufs {
   uint32_t flag_words;  // 32-bit words of bits
   uint32_t *loc;     // working id -> local index inside a subtree block
   uint32_t *cnt;     // per-tile left counts of the device-wide levels
-  uint32_t *tieflag; // per-segment "equal keys straddle the median"
   uint32_t *perm;    // tree position -> working id
 };
 
 Shape make_shape(int64_t n, int leaf);
 SysDesc make_sys(const Shape &sh, uint32_t body_off, uint32_t node_off, uint32_t list_off);
 size_t sort_temp_bytes(uint32_t n, int nsys);
-// per-tile count entries and tie flags the device-wide levels of a forest need
+// per-tile count entries the device-wide levels of a forest need
 size_t level_cnt_entries(const Forest &f);
-size_t tieflag_entries(const Forest &f);
 hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t s);
 // range_ready: b.range already holds the range of `in` (computed by the
 // kick-drift that produced it), so the build skips its own range pass.

@@ -86,6 +86,7 @@ def lib():
         L.orc_simple_sim.restype = C.c_int
         L.orc_direct.argtypes = [C.c_int64, _d, _d, _d, _d, C.c_int64, _i64, _d, _d, _d,
                                  _d, _d, _d]
+        L.orc_direct_comp.argtypes = L.orc_direct.argtypes
         L.orc_quickstat_index.argtypes = [_i64, C.c_int64, C.c_int64, _d, C.c_uint64]
         L.orc_nodes_needed.argtypes = [C.c_int64, C.c_int, C.c_int]
         L.orc_nodes_needed.restype = C.c_int64
@@ -120,6 +121,8 @@ def ref_lib(kind: str = "seq"):
         L.ref_cpu_sample.argtypes = [C.c_int64, _d, _d, _d, _d, _d, _d, _d, C.c_int64, _i64,
                                      C.c_int, C.c_double, _d, _d]
         L.ref_max_parts.restype = C.c_int64
+        L.ref_simple_sim_timed.argtypes = [C.c_int64] + [_d] * 7 + [C.c_double, C.c_int64,
+                                                                   C.c_int, _d]
         _REF[kind] = L
     return _REF[kind]
 
@@ -255,13 +258,16 @@ def adj_backward(tape: Tape, lp, lv):
     return np.stack(out[0:3], 1), np.stack(out[3:6], 1), out[6]
 
 
-def direct(b: dict, idx):
+def direct(b: dict, idx, compensated: bool = False):
+    """FP64 direct sums (calc_pp_accel terms) for the bodies idx and the sums
+    of |a_ij|; compensated: Neumaier-summed, exact to about one rounding."""
     L = lib()
     x, y, z, m = (_f(b, k) for k in "xyzm")
     idx = np.ascontiguousarray(idx, dtype=np.int64)
     k = len(idx)
     a = [np.empty(k) for _ in range(6)]
-    L.orc_direct(len(x), _d_(x), _d_(y), _d_(z), _d_(m), k, _i_(idx), *[_d_(v) for v in a])
+    fn = L.orc_direct_comp if compensated else L.orc_direct
+    fn(len(x), _d_(x), _d_(y), _d_(z), _d_(m), k, _i_(idx), *[_d_(v) for v in a])
     return np.stack(a[:3], axis=1), np.stack(a[3:], axis=1)
 
 
@@ -319,6 +325,21 @@ def ref_accel(b: dict, seed: int = 1, kind: str = "seq"):
     a = [np.empty(n) for _ in range(3)]
     R.ref_accel(n, _d_(x), _d_(y), _d_(z), _d_(m), seed, *[_d_(v) for v in a])
     return np.stack(a, axis=1)
+
+
+def ref_simple_sim_timed(b: dict, dt=1e-3, steps=1, threads=0):
+    """The reference's Parallel/CVersion simple_sim (OpenMP) run whole on a
+    copy of b; returns (final state, wall seconds of the call), or None when
+    the reference build is absent."""
+    R = ref_lib("par")
+    if R is None:
+        return None
+    arrs = {k: _f(b, k).copy() for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    t = np.zeros(1)
+    R.ref_simple_sim_timed(len(arrs["x"]), *[_d_(arrs[k]) for k in ("x", "y", "z", "vx", "vy",
+                                                                    "vz", "m")],
+                           dt, steps, threads, _d_(t))
+    return arrs, float(t[0])
 
 
 def ref_sim(b: dict, dt=1e-3, steps=1, seed=1, kind="seq"):

@@ -409,3 +409,36 @@ void orc_direct(int64_t n, const double *x, const double *y, const double *z,
     if (absx) { absx[s] = ab[0]; absy[s] = ab[1]; absz[s] = ab[2]; }
   }
 }
+
+/* Neumaier's compensated summation: s + c holds the running sum to about
+ * one rounding of the result, whatever the order or the cancellation. */
+static void neumaier_add(double *s, double *c, double v) {
+  const double t = *s + v;
+  if (fabs(*s) >= fabs(v))
+    *c += (*s - t) + v;
+  else
+    *c += (v - t) + *s;
+  *s = t;
+}
+
+void orc_direct_comp(int64_t n, const double *x, const double *y, const double *z,
+                     const double *m, int64_t k, const int64_t *idx, double *ax,
+                     double *ay, double *az, double *absx, double *absy, double *absz) {
+  acc_ctx a = {NULL, x, y, z, m, 0.0, 0, {0, 0, 0}};
+  for (int64_t s = 0; s < k; ++s) {
+    int64_t i = idx[s];
+    double acc[3] = {0.0, 0.0, 0.0}, cc[3] = {0.0, 0.0, 0.0};
+    double ab[3] = {0.0, 0.0, 0.0}, cb[3] = {0.0, 0.0, 0.0};
+    for (int64_t j = 0; j < n; ++j) {
+      if (j == i) continue;
+      double pp[3];
+      pp_accel(&a, i, j, pp);
+      for (int c = 0; c < 3; ++c) {
+        neumaier_add(&acc[c], &cc[c], pp[c]);
+        neumaier_add(&ab[c], &cb[c], fabs(pp[c]));
+      }
+    }
+    ax[s] = acc[0] + cc[0]; ay[s] = acc[1] + cc[1]; az[s] = acc[2] + cc[2];
+    if (absx) { absx[s] = ab[0] + cb[0]; absy[s] = ab[1] + cb[1]; absz[s] = ab[2] + cb[2]; }
+  }
+}

@@ -125,6 +125,14 @@ void orc_direct(int64_t n, const double *x, const double *y, const double *z,
                 double *ay, double *az, double *absx, double *absy,
                 double *absz);
 
+/* The same terms as orc_direct, summed with Neumaier's compensated
+ * summation: the sums are exact to about one rounding, so they measure the
+ * summation error of any other order of the same terms. */
+void orc_direct_comp(int64_t n, const double *x, const double *y, const double *z,
+                     const double *m, int64_t k, const int64_t *idx, double *ax,
+                     double *ay, double *az, double *absx, double *absy,
+                     double *absz);
+
 /* Kick-drift as simple_sim does it (array_kd_tree.rs:203-216). */
 void orc_kick_drift(int64_t n, double *x, double *y, double *z, double *vx,
                     double *vy, double *vz, const double *ax, const double *ay,

@@ -202,3 +202,45 @@ int ref_cpu_sample(int64_t n, const double *x, const double *y,
   FREE_ARRAY(ix);
   return 0;
 }
+
+/* CPU baseline, whole steps: the reference's own simple_sim
+ * (Parallel/CVersion/src/kdtree.c:221-260: per step identity indices,
+ * build_tree on one thread, calc_accel of every body and the kick-drift as
+ * OpenMP parallel-for loops) on n bodies for `steps` steps, `threads` OpenMP
+ * threads.  The state is updated in place; t_out[0] = wall seconds of the
+ * simple_sim call alone (no particle conversion). */
+int ref_simple_sim_timed(int64_t n, double *x, double *y, double *z, double *vx,
+                         double *vy, double *vz, const double *m, double dt,
+                         int64_t steps, int threads, double *t_out) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#endif
+  srand(12345u);
+  Particle_array_t ps = make_particles(n, x, y, z, m);
+  for (int64_t i = 0; i < n; ++i) {
+    ps.ptr[i].v[0] = vx[i];
+    ps.ptr[i].v[1] = vy[i];
+    ps.ptr[i].v[2] = vz[i];
+  }
+#ifdef _OPENMP
+  double t0 = omp_get_wtime();
+#else
+  double t0 = 0.0;
+#endif
+  simple_sim(&ps, dt, (int)steps);
+#ifdef _OPENMP
+  t_out[0] = omp_get_wtime() - t0;
+#else
+  t_out[0] = -1.0;
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    x[i] = ps.ptr[i].p[0];
+    y[i] = ps.ptr[i].p[1];
+    z[i] = ps.ptr[i].p[2];
+    vx[i] = ps.ptr[i].v[0];
+    vy[i] = ps.ptr[i].v[1];
+    vz[i] = ps.ptr[i].v[2];
+  }
+  FREE_ARRAY(ps);
+  return 0;
+}

@@ -64,14 +64,14 @@ def test_fp64_accel_sample_matches_oracle(orc, ring):
 def test_mixed_force_error_bounds(orc, ring, theta):
     # the headline error gate at both opening angles of the mixed-precision
     # workload: 4,096 seeded bodies against the FP64 direct sum, whose values
-    # on 16 of them are checked against the oracle's direct sum first
+    # on 16 of them are checked against the oracle's compensated direct sum
     with K.Simulation(N, theta=theta, precision=K.KD_PREC_MIXED) as s:
         s.upload(ring)
         a = s.accel()
         idx = np.sort(np.random.default_rng(12).choice(N, 4096, replace=False))
         ref, absref = s.direct_sum(idx)
     pin = idx[::256]
-    oref, oabs = orc.direct(ring, pin)
+    oref, oabs = orc.direct(ring, pin, compensated=True)
     _, cond = accel_errors(ref[::256], oref, oabs)
     assert cond.max() <= 1e-12, cond.max()
     assert np.isfinite(a).all()

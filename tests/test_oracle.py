@@ -250,3 +250,35 @@ def test_oracle_vs_reference_c(orc, seed):
     a, ab, _ = orc.accel(b, want_abs=True)
     _, cond = accel_errors(a, orc.ref_accel(b, seed=seed), ab)
     assert cond.max() <= 1e-13
+
+
+def test_compensated_direct_sum_is_exactly_rounded(orc):
+    # orc_direct_comp against math.fsum of the same FP64 terms (numpy forms
+    # each term as calc_pp_accel does, array_particle.rs:67-76: no fused
+    # operations, correctly rounded sqrt and divide), which is the correctly
+    # rounded sum; and orc_direct's left-to-right sums within the classic
+    # (n-1) u sum|a_ij| bound of the exact ones
+    import math
+    n = 3000
+    rng = np.random.default_rng(61)
+    r, phi = rng.uniform(1.0, 2.0, n), rng.uniform(0.0, 2 * np.pi, n)
+    b = dict(x=r * np.cos(phi), y=r * np.sin(phi), z=rng.normal(0.0, 1e-3, n),
+             m=np.full(n, 1e-14))
+    for k in "xyz":
+        b[k][0] = 0.0
+    b["m"][0] = 1.0
+    idx = np.array([0, 1, 17, 2999])
+    comp, cabs = orc.direct(b, idx, compensated=True)
+    plain, pabs = orc.direct(b, idx)
+    x, y, z, m = (b[k] for k in "xyzm")
+    for r, i in enumerate(idx):
+        j = np.arange(n) != i
+        dx, dy, dz = x[i] - x[j], y[i] - y[j], z[i] - z[j]
+        dist = np.sqrt(dx * dx + dy * dy + dz * dz)
+        magi = -m[j] / (dist * dist * dist)
+        for c, d in enumerate((dx, dy, dz)):
+            t = magi * d
+            exact = math.fsum(t)
+            assert abs(comp[r, c] - exact) <= 2 * np.spacing(abs(exact)), (i, c)
+            assert abs(cabs[r, c] - math.fsum(np.abs(t))) <= 2 * np.spacing(cabs[r, c])
+            assert abs(plain[r, c] - exact) <= (n - 1) * 2.0 ** -53 * pabs[r, c]
