@@ -324,6 +324,18 @@ def run_select(n, seed, reps):
             "speedup_vs_cpu": t_cpu / float(np.median(times))}
 
 
+def run_child(argv):
+    """Run one leg of this script in a child process; its JSON line, or an
+    error record if the child failed."""
+    import subprocess
+    r = subprocess.run([sys.executable, str(Path(__file__).resolve())] + argv,
+                       capture_output=True, text=True)
+    sys.stderr.write(r.stderr)
+    if r.returncode != 0:
+        return {"error": f"child exited with {r.returncode}", "argv": argv}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 def tree_depth(n, leaf):
     """Levels below the root of the tree of n bodies (every split at floor(len/2))."""
     d, mx = 0, n
@@ -549,11 +561,19 @@ def main():
         log(f"[bench] batch {json.dumps(batch)}")
 
     if not args.no_extra:
+        # the legs that use torch run in a child process of their own (the
+        # parent has created and destroyed hundreds of library contexts by now)
         log(f"[bench] adjoint: n={args.adj_n} steps={args.adj_steps}")
-        extra["adjoint"] = run_adjoint(args.adj_n, args.adj_steps, args.adj_every, args.theta,
-                                       args.leaf, dt, args.seed, args.adj_dirs, args.adj_cpu_n)
+        extra["adjoint"] = run_child(["--only", "adjoint", "--adj-n", str(args.adj_n),
+                                      "--adj-steps", str(args.adj_steps),
+                                      "--adj-every", str(args.adj_every),
+                                      "--adj-dirs", str(args.adj_dirs),
+                                      "--adj-cpu-n", str(args.adj_cpu_n),
+                                      "--theta", str(args.theta), "--leaf", str(args.leaf),
+                                      "--seed", str(args.seed)])
         log("[bench] selection leg")
-        extra["select"] = run_select(args.select_n, args.seed, 3)
+        extra["select"] = run_child(["--only", "select", "--select-n", str(args.select_n),
+                                     "--seed", str(args.seed)])
         log(f"[bench] select {extra['select']}")
         log("[bench] build-only leg")
         extra["build_only"] = run_build_only(n, args.build_leaf, args.seed, 5)

@@ -707,21 +707,29 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_count(ListBufs LB, TreeDe
   }
 }
 
-// Exclusive scan over the T tile counts of one (list, segment).
-__global__ __launch_bounds__(256) void k_level_scan(uint32_t *cnt, uint32_t T,
-                                                    const SysDesc *sys, uint32_t level) {
-  __shared__ uint32_t sh[4];
+// Exclusive scan over the T tile counts of one (list, segment): thread i
+// sums a run of ceil(T / NT) consecutive counts, one block scan joins the
+// runs, and each thread writes its run's prefixes (so a level-0 row of
+// ~10^4 tiles takes one pass instead of one per 256 counts).  Rows of at
+// most SCAN_SMALL_T tiles (the deeper levels) take one wavefront each.
+constexpr uint32_t SCAN_SMALL_T = 1024;
+template <int NT>
+__global__ __launch_bounds__(NT) void k_level_scan(uint32_t *cnt, uint32_t T, const SysDesc *sys,
+                                                   uint32_t level) {
+  __shared__ uint32_t sh[NT / 64];
   if (level >= sys[blockIdx.y].sub_level) return;
   const uint32_t rows = 3u << level;
   uint32_t *row = cnt + ((size_t)blockIdx.y * rows + blockIdx.x) * T;
-  uint32_t carry = 0;
-  for (uint32_t b = 0; b < T; b += 256) {
-    uint32_t i = b + threadIdx.x;
-    uint32_t v = i < T ? row[i] : 0;
-    uint32_t tot;
-    uint32_t ex = block_excl_sum<256>(v, sh, tot);
-    if (i < T) row[i] = carry + ex;
-    carry += tot;
+  const uint32_t per = (T + NT - 1) / NT;
+  const uint32_t b = min(threadIdx.x * per, T), e = min(b + per, T);
+  uint32_t sum = 0;
+  for (uint32_t i = b; i < e; ++i) sum += row[i];
+  uint32_t tot;
+  uint32_t run = block_excl_sum<NT>(sum, sh, tot);
+  for (uint32_t i = b; i < e; ++i) {
+    const uint32_t v = row[i];
+    row[i] = run;
+    run += v;
   }
 }
 
@@ -1394,7 +1402,10 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
     k_level_pack<<<(b.flag_words + 255) / 256, 256, 0, s>>>(b.flags, b.bits, b.flag_words);
     k_level_count<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.bits,
                                                              b.cnt);
-    k_level_scan<<<dim3(3 * nseg, ny), 256, 0, s>>>(b.cnt, T, f.sys, lev);
+    if (T > SCAN_SMALL_T)
+      k_level_scan<1024><<<dim3(3 * nseg, ny), 1024, 0, s>>>(b.cnt, T, f.sys, lev);
+    else
+      k_level_scan<64><<<dim3(3 * nseg, ny), 64, 0, s>>>(b.cnt, T, f.sys, lev);
     k_level_scatter<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.bits,
                                                                b.cnt);
   }
