@@ -200,6 +200,21 @@ int kd_quickstat_index_device(int64_t len, int64_t *indices, int64_t goal,
                               const double *vals, int64_t nvals, void *workspace,
                               int64_t workspace_bytes);
 
+/* The same selection with no workspace argument and no allocation of any
+ * kind: no counterpart in the reference.  It is for callers that cannot
+ * allocate device memory (code under stream capture, serving loops, hosts
+ * that forbid allocation).  The workspace is a fixed device array of
+ * KD_NOALLOC_SCRATCH_BYTES inside the library, one per device, held until the
+ * process ends; len above KD_NOALLOC_SELECT_MAX_LEN returns KD_ERR_CAPACITY.
+ * Every call in a process on one device shares that array and leaves its
+ * contents behind: calls that overlap on one device (from two host threads)
+ * race and must be serialised by the caller.  Runs on the null stream and
+ * returns after it completes. */
+#define KD_NOALLOC_SCRATCH_BYTES (64ll << 20)
+#define KD_NOALLOC_SELECT_MAX_LEN 4000000ll
+int kd_quickstat_index_device_noalloc(int64_t len, int64_t *indices, int64_t goal,
+                                      const double *vals, int64_t nvals);
+
 /* ---- batches: many independent systems in one context ----
  * A batch holds nsys independent systems of n[i] bodies each and runs
  * simple_sim (Parallel/RustVersion/src/array_kd_tree.rs:623-664) on all of

@@ -96,6 +96,8 @@ SIGNATURES = {
     "kd_direct_sum": (C.c_int, [C.c_void_p, C.c_int64, _i64] + [_d] * 6),
     "kd_quickstat_index": (C.c_int, [C.c_int64, _i64, C.c_int64, _d, C.c_int64]),
     "kd_quickstat_workspace_bytes": (C.c_int64, [C.c_int64]),
+    "kd_quickstat_index_device_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_int64,
+                                                    C.c_void_p, C.c_int64]),
     "kd_quickstat_index_device": (C.c_int, [C.c_int64, C.c_void_p, C.c_int64, C.c_void_p,
                                             C.c_int64, C.c_void_p, C.c_int64]),
     "kd_batch_create": (C.c_int, [C.c_int64, _i64, _P(KdParams), _P(C.c_void_p)]),

@@ -3,6 +3,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1719,6 +1720,62 @@ extern "C" int kd_quickstat_index_device(int64_t len, int64_t *indices, int64_t 
   if (workspace_bytes < (int64_t)select_workspace_bytes(len))
     return fail(KD_ERR_CAPACITY, "workspace smaller than kd_quickstat_workspace_bytes(len)");
   hipError_t e = launch_select(indices, vals, len, nvals, goal, workspace, nullptr);
+  if (e == hipErrorInvalidValue) return fail(KD_ERR_ARG, "an index is outside [0, nvals)");
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) return fail(KD_ERR_HIP, "selection: %s", hipGetErrorString(e));
+  return KD_OK;
+}
+
+// ---- selection without allocation ----
+// kd_quickstat_index_device_noalloc keeps the selection's workspace (sort
+// keys, the partitioned copy, per-block counts, the selection state) in this
+// file-scope device array instead of a caller workspace.  Each device holds
+// its own copy for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device (the
+// array's name is a host-side stand-in, not a device address).
+static __device__ __align__(256) unsigned char scratch[KD_NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+// The address of `scratch` on the current device, looked up once per device.
+static int scratch_on_device(void **out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+    return fail(KD_ERR_NODEVICE, "device %d is beyond the %d devices served", dev,
+                SCRATCH_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(KD_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return KD_OK;
+}
+
+extern "C" int kd_quickstat_index_device_noalloc(int64_t len, int64_t *indices, int64_t goal,
+                                                 const double *vals, int64_t nvals) {
+  if (len < 1 || !indices || !vals || nvals < 1)
+    return fail(KD_ERR_ARG, "bad argument to kd_quickstat_index_device_noalloc");
+  if (goal < 0 || goal >= len) return fail(KD_ERR_ARG, "goal must be in [0, len)");
+  if (len > KD_NOALLOC_SELECT_MAX_LEN)
+    return fail(KD_ERR_CAPACITY,
+                "len %lld is above KD_NOALLOC_SELECT_MAX_LEN (%lld), the most the fixed "
+                "device workspace holds; use kd_quickstat_index_device with a workspace",
+                (long long)len, (long long)KD_NOALLOC_SELECT_MAX_LEN);
+  if (select_workspace_bytes(len) > KD_NOALLOC_SCRATCH_BYTES)
+    return fail(KD_ERR_CAPACITY, "workspace of len %lld exceeds the fixed device workspace",
+                (long long)len);
+  void *ws = nullptr;
+  int rc = scratch_on_device(&ws);
+  if (rc) return rc;
+  hipError_t e = launch_select(indices, vals, len, nvals, goal, ws, nullptr);
   if (e == hipErrorInvalidValue) return fail(KD_ERR_ARG, "an index is outside [0, nvals)");
   if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) return fail(KD_ERR_HIP, "selection: %s", hipGetErrorString(e));
