@@ -29,6 +29,9 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+# the reference C baseline runs OpenMP in this process: idle worker threads
+# sleep instead of spinning on the host cores the other legs use
+os.environ.setdefault("OMP_WAIT_POLICY", "PASSIVE")
 
 from multilanguagekdtree_amd import kdtree as K  # noqa: E402
 
