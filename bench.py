@@ -386,9 +386,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     # whole steps of the reference C simple_sim on the headline bodies (one
-    # 10 M step takes about a minute on 16 host cores) and at 1 M bodies
+    # 10 M step takes about a minute on 16 host cores) and the 10 steps of the
+    # 1 M FP64 workload (about 25 s)
     ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--cpu-config1-steps", type=int, default=3)
+    ap.add_argument("--cpu-config1-steps", type=int, default=10)
     ap.add_argument("--cpu-repeats", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error", action="store_true")
@@ -457,8 +458,13 @@ def main():
         # the reference C simple_sim alone: median of --cpu-repeats runs
         out = {}
         for nn, st in ((n, args.cpu_steps), (args.config1_n, args.config1_steps)):
+            if st <= 0:
+                continue
             b = K.ring(nn, seed=args.seed)
-            runs = [cpu_baseline(b, nn, st, threads, dt) for _ in range(args.cpu_repeats)]
+            runs = []
+            for rep in range(args.cpu_repeats):
+                runs.append(cpu_baseline(b, nn, st, threads, dt))
+                log(f"[bench] cpu n={nn} steps={st} run {rep + 1}: {runs[-1]['seconds']:.2f} s")
             vals = sorted(r["value"] for r in runs)
             out[f"n{nn}_steps{st}"] = dict(runs[0], value=vals[len(vals) // 2],
                                           values=vals, repeats=args.cpu_repeats)
@@ -535,11 +541,20 @@ def main():
         extra["theta_0.5"] = step_leg(bodies, n, args.steps, args.warmup, 0.5, args.leaf, dt)
     del bodies
 
-    cpu = None
+    # the CPU legs run in a child process of their own: inside this process,
+    # after the GPU legs, the same reference code measured 1.7 to 5 times
+    # slower than in a fresh process
+    cpu = c1 = None
     if not args.no_cpu_baseline:
+        st1 = 0 if args.no_extra else args.cpu_config1_steps
         log(f"[bench] CPU baseline: reference C simple_sim, {threads} threads, "
-            f"{args.cpu_steps} whole step(s) at n={n}")
-        cpu = cpu_baseline(K.ring(n, seed=args.seed), n, args.cpu_steps, threads, dt)
+            f"{args.cpu_steps} whole step(s) at n={n}"
+            + (f", {st1} at n={args.config1_n}" if st1 else ""))
+        rc = run_child(["--only", "cpu", "--n", str(n), "--cpu-steps", str(args.cpu_steps),
+                        "--config1-n", str(args.config1_n), "--config1-steps", str(st1),
+                        "--cpu-threads", str(threads), "--seed", str(args.seed)])
+        cpu = rc.get(f"n{n}_steps{args.cpu_steps}")
+        c1 = rc.get(f"n{args.config1_n}_steps{st1}")
         log(f"[bench] cpu {cpu}")
 
     if not args.no_extra:
@@ -548,8 +563,7 @@ def main():
         b1 = K.ring(n1, seed=args.seed)
         extra["config1_fp64"] = step_leg(b1, n1, args.config1_steps, args.warmup, args.theta,
                                          args.leaf, dt, precs=("fp64",), err=False)
-        if not args.no_cpu_baseline and args.cpu_config1_steps > 0:
-            c1 = cpu_baseline(b1, n1, args.cpu_config1_steps, threads, dt)
+        if c1 is not None:
             extra["config1_fp64"]["cpu_reference"] = c1
             extra["config1_fp64"]["speedup_vs_cpu"] = (
                 extra["config1_fp64"]["fp64"]["value"] / c1["value"])
@@ -579,7 +593,9 @@ def main():
                                      "--seed", str(args.seed)])
         log(f"[bench] select {extra['select']}")
         log("[bench] build-only leg")
-        extra["build_only"] = run_build_only(n, args.build_leaf, args.seed, 5)
+        extra["build_only"] = run_child(["--only", "build", "--n", str(n),
+                                         "--build-leaf", str(args.build_leaf),
+                                         "--seed", str(args.seed)])
         log(f"[bench] build only {extra['build_only']}")
 
     out = {
