@@ -12,7 +12,10 @@ namespace kd {
 // launch shape (SUB_THREADS threads x SUB_ITEMS consecutive positions).
 // 2048 (37 KB of LDS a workgroup): against 4096 (74 KB), one more device-wide
 // level but more, smaller workgroups; at 10 M bodies a build takes 4.36 ms
-// against 4.40, and a 512-system batch step 29.9 ms against 32.0.
+// against 4.40, and a 512-system batch step 29.9 ms against 32.0.  Other
+// shapes, at 10 M bodies: 4.06 ms for 512 x 4, against 4.17 for 256 x 4 and
+// 4.46 for 512 x 2 (segments of 1024, one more device-wide level), 4.19 for
+// 256 x 8 and 5.13 for 1024 x 2.
 #ifndef KD_SUB_THREADS
 #define KD_SUB_THREADS 512
 #endif
