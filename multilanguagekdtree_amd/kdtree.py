@@ -182,32 +182,31 @@ class Simulation:
 
 def _export(call, n: int) -> Tree:
     """Run an export entry point twice: once for the length, once to fill."""
-    if True:
-        v = KdTreeView()
-        v.n_nodes = 0
-        rc = call(C.byref(v))
-        if rc != _lib.KD_ERR_CAPACITY:
-            check(rc)
-        nn = v.n_nodes
-        a = dict(
-            is_leaf=np.zeros(nn, np.int32), split_dim=np.zeros(nn, np.int32),
-            num_parts=np.zeros(nn, np.int64), first=np.zeros(nn, np.int64),
-            split_val=np.zeros(nn), m=np.zeros(nn), cmx=np.zeros(nn),
-            cmy=np.zeros(nn), cmz=np.zeros(nn), size=np.zeros(nn),
-            left=np.zeros(nn, np.int64), right=np.zeros(nn, np.int64),
-            perm=np.zeros(n, np.int64))
-        v.n_nodes = nn
-        v.is_leaf = i32ptr(a["is_leaf"])
-        v.split_dim = i32ptr(a["split_dim"])
-        for k in ("num_parts", "first", "left", "right", "perm"):
-            setattr(v, k, iptr(a[k]))
-        for k in ("split_val", "m", "cmx", "cmy", "cmz", "size"):
-            setattr(v, k, dptr(a[k]))
-        check(call(C.byref(v)))
-        return Tree(a["is_leaf"], a["split_dim"], a["num_parts"], a["first"],
-                    a["split_val"], a["m"],
-                    np.stack([a["cmx"], a["cmy"], a["cmz"]], axis=1), a["size"],
-                    a["left"], a["right"], a["perm"])
+    v = KdTreeView()
+    v.n_nodes = 0
+    rc = call(C.byref(v))
+    if rc != _lib.KD_ERR_CAPACITY:
+        check(rc)
+    nn = v.n_nodes
+    a = dict(
+        is_leaf=np.zeros(nn, np.int32), split_dim=np.zeros(nn, np.int32),
+        num_parts=np.zeros(nn, np.int64), first=np.zeros(nn, np.int64),
+        split_val=np.zeros(nn), m=np.zeros(nn), cmx=np.zeros(nn),
+        cmy=np.zeros(nn), cmz=np.zeros(nn), size=np.zeros(nn),
+        left=np.zeros(nn, np.int64), right=np.zeros(nn, np.int64),
+        perm=np.zeros(n, np.int64))
+    v.n_nodes = nn
+    v.is_leaf = i32ptr(a["is_leaf"])
+    v.split_dim = i32ptr(a["split_dim"])
+    for k in ("num_parts", "first", "left", "right", "perm"):
+        setattr(v, k, iptr(a[k]))
+    for k in ("split_val", "m", "cmx", "cmy", "cmz", "size"):
+        setattr(v, k, dptr(a[k]))
+    check(call(C.byref(v)))
+    return Tree(a["is_leaf"], a["split_dim"], a["num_parts"], a["first"],
+                a["split_val"], a["m"],
+                np.stack([a["cmx"], a["cmy"], a["cmz"]], axis=1), a["size"],
+                a["left"], a["right"], a["perm"])
 
 
 class Batch:

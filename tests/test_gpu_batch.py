@@ -62,6 +62,25 @@ def test_fp64_accel_matches_oracle(orc, bodies):
         assert cond.max() <= 1e-12, (i, cond.max())
 
 
+@pytest.mark.parametrize("leaf,theta", [(2, 0.3), (8, 0.5), (32, 0.3)])
+def test_other_leaf_sizes_and_angles_match_oracle(orc, bodies, leaf, theta):
+    # the forest's trees (both layouts) and FP64 accelerations at the leaf
+    # sizes and opening angle the one-system tests also cover
+    with K.Batch(SIZES, theta=theta, leaf_size=leaf, precision=K.KD_PREC_FP64) as bt:
+        bt.upload(bodies)
+        a = bt.accel()
+        for i in range(len(SIZES)):
+            b = bt.system(bodies, i)
+            for layout in (K.KD_LAYOUT_COMPACT, K.KD_LAYOUT_PADDED):
+                compare_trees(bt.export_tree(i, layout), orc.build_tree(b, leaf, layout),
+                              pscale(b))
+    for i in (0, 2, 3, 5, 9, 11):
+        b = bt.system(bodies, i)
+        ref, absref, _ = orc.accel(b, theta=theta, leaf=leaf, want_abs=True)
+        _, cond = accel_errors(a[bt.offsets[i]:bt.offsets[i + 1]], ref, absref)
+        assert cond.max() <= 1e-12, (i, cond.max())
+
+
 @pytest.mark.parametrize("shape", [(256, 0), (64, 8), (512, 32), (128, 16)])
 def test_launch_shapes_agree(bodies, shape):
     with K.Batch(SIZES) as bt:
