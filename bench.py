@@ -524,7 +524,9 @@ def main():
         issue = valu["valu_insts_per_wave"] * waves / (r["traverse_ms"] * 1e-3 * CUS * CLOCK_HZ)
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
                     frac=round(achieved / peak, 4), traffic=traffic,
-                    kernel=f"k_traverse_{'mixed' if main_name == 'mixed' else 'f64'}",
+                    # the walk's kernel as rocprofv3 names it (profiles/step_kernels_*.md)
+                    kernel=("kd::k_walk<kd::F32, false, false>" if main_name == "mixed"
+                            else "kd::k_walk_f64<false>"),
                     kernel_ms=round(r["traverse_ms"], 3),
                     hbm_gbps=(None if traffic is None
                               else round(traffic / (r["traverse_ms"] * 1e-3) / 1e9, 1)),
