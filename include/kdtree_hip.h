@@ -165,9 +165,19 @@ int kd_counters(kd_ctx *ctx, int64_t out[5]);
 /* Launch-shape knobs of the kernels (no counterpart in the reference):
  * "leaf_predicated" 0/1 -- compute whole leaf batches with predication
  * instead of branching per body; "fp64_batched_walk" 0/1 -- run the FP64
- * path on the batched walk of the mixed paths.  Unknown keys return
- * KD_ERR_ARG. */
+ * path on the batched walk of the mixed paths; "lazy_dimension" 0/1 (default
+ * 1) -- let the build skip the sort and the device-wide partitions of a
+ * dimension that the previous full build found could not be split at those
+ * levels (a thin system's third dimension); every build checks that this
+ * holds and builds again in full when it does not, so trees are the same
+ * either way.  Unknown keys return KD_ERR_ARG. */
 int kd_set_tuning(kd_ctx *ctx, const char *key, int64_t value);
+
+/* Reads a knob of kd_set_tuning, or the lazy-dimension state: "lazy_next"
+ * (the dimension the next build skips, 3 for none), "lazy_builds" (builds
+ * that skipped one) and "lazy_redos" (of those, builds repeated in full).
+ * No counterpart in the reference. */
+int kd_get_tuning(kd_ctx *ctx, const char *key, int64_t *value);
 
 /* With params.profile set, kd_step accumulates per phase the HIP-event time
  * on the context's stream: out = {build ms, traversal ms, kick-drift ms,

@@ -91,6 +91,7 @@ SIGNATURES = {
     "kd_sync": (C.c_int, [C.c_void_p]),
     "kd_counters": (C.c_int, [C.c_void_p, _i64]),
     "kd_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "kd_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, _i64]),
     "kd_timings": (C.c_int, [C.c_void_p, _d]),
     "kd_reset_timings": (C.c_int, [C.c_void_p]),
     "kd_direct_sum": (C.c_int, [C.c_void_p, C.c_int64, _i64] + [_d] * 6),

@@ -79,6 +79,16 @@ struct kd_ctx {
   SysDesc *dsys = nullptr;
   uint32_t *sob = nullptr;  // system of every 64 body positions
   uint32_t extra_recs = 0;  // walk records built beyond the precision's own
+  // lazy dimension (one system, never captured): the build skips the sort and
+  // the device-wide partitions of a dimension that a full build reported
+  // could not be split there; a build that finds otherwise is repeated in
+  // full, and lazy builds pause for LAZY_PAUSE builds
+  static constexpr int LAZY_PAUSE = 16;
+  bool lazy_allowed = false;  // single-system context
+  bool lazy_on = true;        // kd_set_tuning("lazy_dimension", 0) turns it off
+  int lazy_dim = 3;           // dimension the next build skips (3: none)
+  int lazy_pause = 0;
+  int64_t lazy_builds = 0, lazy_redos = 0;
 };
 
 template <typename T>
@@ -228,6 +238,8 @@ int kd_destroy(kd_ctx *c) {
     if (c->b.join[i]) (void)hipEventDestroy(c->b.join[i]);
   }
   if (c->b.fork) (void)hipEventDestroy(c->b.fork);
+  if (c->b.ev_viol) (void)hipEventDestroy(c->b.ev_viol);
+  if (c->b.viol_host) (void)hipHostFree(c->b.viol_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return KD_OK;
@@ -280,6 +292,10 @@ static int create_impl(kd_ctx *c, int64_t cap, const Sizing &z) {
     HIP_TRY(hipEventCreateWithFlags(&c->b.join[i], hipEventDisableTiming));
   }
   HIP_TRY(hipEventCreateWithFlags(&c->b.fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&c->b.ev_viol, hipEventDisableTiming));
+  ALLOC(c->allocs, c->b.viol, 1);
+  HIP_TRY(hipHostMalloc((void **)&c->b.viol_host, sizeof(uint32_t), hipHostMallocDefault));
+  *c->b.viol_host = 0;
   ALLOC(c->allocs, c->b.range, 8);
   c->b.flag_words = (uint32_t)((N + 31) / 32);
   ALLOC(c->allocs, c->b.flags, 32 * (size_t)c->b.flag_words);
@@ -333,6 +349,7 @@ int kd_create(int64_t cap_n, const kd_params *params, kd_ctx **out) {
     g_err = msg;
     return rc;
   }
+  c->lazy_allowed = true;
   *out = c;
   return KD_OK;
 }
@@ -449,8 +466,46 @@ static uint32_t records_for(const kd_ctx *c) {
 static int build_impl(kd_ctx *c, bool range_ready = false) {
   int nxt = c->cur ^ 1;
   c->tree.recs = records_for(c);
+  // a single-system context learns after each build (from the flags the
+  // device-wide levels report) whether the next may skip a dimension; the
+  // host waits for those flags while the GPU runs the subtree builder
+  const bool report = c->lazy_allowed && c->lazy_on;
+  const int lazy = report && c->lazy_pause == 0 ? c->lazy_dim : 3;
   HIP_TRY(launch_build(c->forest, c->params.leaf_size, c->st[c->cur], c->st[nxt], c->tree, c->b,
-                       c->pos64, c->pos32, c->params.theta, c->stream, range_ready));
+                       c->pos64, c->pos32, c->params.theta, c->stream, range_ready, lazy,
+                       report));
+  if (report) {
+    HIP_TRY(hipEventSynchronize(c->b.ev_viol));
+    uint32_t unsafe = *c->b.viol_host;
+    bool full = lazy == 3;  // a full build reports every dimension
+    if (lazy < 3) {
+      ++c->lazy_builds;
+      if (unsafe & (1u << lazy)) {
+        // the skipped dimension could have been a device-wide split: the
+        // tree is not exact, so build again from the same input with every
+        // list (the range of `in` is in b.range by now)
+        ++c->lazy_redos;
+        HIP_TRY(launch_build(c->forest, c->params.leaf_size, c->st[c->cur], c->st[nxt], c->tree,
+                             c->b, c->pos64, c->pos32, c->params.theta, c->stream, true, 3,
+                             true));
+        HIP_TRY(hipEventSynchronize(c->b.ev_viol));
+        unsafe = *c->b.viol_host;
+        full = true;
+        c->lazy_pause = kd_ctx::LAZY_PAUSE;
+      }
+    } else if (c->lazy_pause > 0) {
+      --c->lazy_pause;
+    }
+    if (full) {
+      // the next lazy dimension: the first of z, x, y this build found safe
+      c->lazy_dim = 3;
+      for (int d : {2, 0, 1})
+        if (!(unsafe & (1u << d))) {
+          c->lazy_dim = d;
+          break;
+        }
+    }
+  }
   c->cur = nxt;
   c->has_tree = true;
   return KD_OK;
@@ -588,7 +643,33 @@ int kd_set_tuning(kd_ctx *c, const char *key, int64_t value) {
     c->traverse_variant = value ? (c->traverse_variant | 4) : (c->traverse_variant & ~4);
     return KD_OK;
   }
+  if (strcmp(key, "lazy_dimension") == 0) {
+    c->lazy_on = value != 0;
+    c->lazy_dim = 3;  // learnt again from the next full build
+    c->lazy_pause = 0;
+    return KD_OK;
+  }
   return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
+}
+
+int kd_get_tuning(kd_ctx *c, const char *key, int64_t *value) {
+  if (!c || !key || !value) return fail(KD_ERR_ARG, "NULL argument");
+  if (strcmp(key, "leaf_predicated") == 0) {
+    *value = (c->traverse_variant & 2) ? 1 : 0;
+  } else if (strcmp(key, "fp64_batched_walk") == 0) {
+    *value = (c->traverse_variant & 4) ? 1 : 0;
+  } else if (strcmp(key, "lazy_dimension") == 0) {
+    *value = c->lazy_allowed && c->lazy_on ? 1 : 0;
+  } else if (strcmp(key, "lazy_next") == 0) {
+    *value = c->lazy_allowed && c->lazy_on && c->lazy_pause == 0 ? c->lazy_dim : 3;
+  } else if (strcmp(key, "lazy_builds") == 0) {
+    *value = c->lazy_builds;
+  } else if (strcmp(key, "lazy_redos") == 0) {
+    *value = c->lazy_redos;
+  } else {
+    return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
+  }
+  return KD_OK;
 }
 
 int kd_timings(kd_ctx *c, double out[4]) {

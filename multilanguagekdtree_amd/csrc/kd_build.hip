@@ -557,16 +557,37 @@ __device__ void tie_fix(uint32_t *ls, uint32_t *tmp, const double *key,
   for (uint32_t p = r0 + threadIdx.x; p < r1; p += NT) ls[p] = tmp[p];
 }
 
+// Split dimension of extents e: the largest, ties to the lower dimension
+// (array_kd_tree.rs:91-96, strict '>').
+__device__ __forceinline__ int split_dim_of(const double e[3]) {
+  int sd = 0;
+  if (e[1] > e[sd]) sd = 1;
+  if (e[2] > e[sd]) sd = 2;
+  return sd;
+}
+
 // The split of every segment of a device-wide level, one block per segment:
 // bbox from the ends of the three sorted lists, split dimension with the
 // reference's strict '>' (array_kd_tree.rs:91-96), split value = key at mid
 // (:125) and the children's list-buffer masks; if equal keys straddle the
 // median, the block orders them (tie_fix).
+//
+// Lazy dimension (lazy < 3): its list is not kept per segment, so its
+// extent is known only to be at most its global extent g (from range; a
+// segment's extent hi - lo cannot exceed hi_all - lo_all, and rounding keeps
+// that order).  The split dimension only grows toward `lazy` as its extent
+// grows, so if the dimension the rule picks with the lazy extent set to g is
+// not `lazy`, the rule picks the same dimension for every extent in [0, g];
+// otherwise the segment sets bit `lazy` of *viol and the build is not exact.
+// With viol set and lazy == 3, the block reports every dimension d that the
+// rule would pick with its extent raised to its global extent.
 constexpr int SPLIT_THREADS = 256;
 __global__ __launch_bounds__(SPLIT_THREADS) void k_level_split(ListBufs LB, Coords P,
                                                                TreeDev t0, const SysDesc *sys,
                                                                const uint32_t *oid,
-                                                               uint32_t level) {
+                                                               uint32_t level,
+                                                               const unsigned long long *range,
+                                                               int lazy, uint32_t *viol) {
   const SysDesc S = sys[blockIdx.y];
   if (level >= S.sub_level) return;
   const uint32_t k = blockIdx.x;
@@ -576,17 +597,37 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_level_split(ListBufs LB, Coor
   const uint32_t m = t.lmask[h];
   __shared__ int sd_s, tie_s;
   if (threadIdx.x == 0) {
-    double ext[3];
+    double ext[3], glob[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
+      glob[d] = ordkey_inv(range[2 * d + 1]) - ordkey_inv(range[2 * d]);
+      if (d == lazy) {
+        ext[d] = glob[d];
+        continue;
+      }
       const uint32_t *ld = LB.in(m, d);
       double lo = P.c[d][ld[s]];
       double hi = P.c[d][ld[s + c - 1]];
       ext[d] = hi - lo;
     }
-    int sd = 0;
-    if (ext[1] > ext[sd]) sd = 1;
-    if (ext[2] > ext[sd]) sd = 2;
+    int sd;
+    if (lazy < 3) {
+      if (split_dim_of(ext) == lazy) atomicOr(viol, 1u << lazy);
+      ext[lazy] = -1.0;
+      sd = split_dim_of(ext);
+    } else {
+      sd = split_dim_of(ext);
+      if (viol != nullptr) {
+        uint32_t unsafe = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          double e[3] = {ext[0], ext[1], ext[2]};
+          e[d] = glob[d];
+          unsafe |= split_dim_of(e) == d ? 1u << d : 0u;
+        }
+        if (unsafe) atomicOr(viol, unsafe);
+      }
+    }
     const uint32_t mid = s + c / 2;
     const double kv = P.c[sd][LB.in(m, sd)[mid]];
     const double kp = P.c[sd][LB.in(m, sd)[mid - 1]];
@@ -661,7 +702,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_count(ListBufs LB, TreeDe
                                                              uint32_t level,
                                                              uint32_t T,
                                                              const uint32_t *__restrict__ bits,
-                                                             uint32_t *cnt) {
+                                                             uint32_t *cnt, int lazy) {
   __shared__ uint32_t sh[LVL_THREADS / 64];
   const SysDesc S = sys[blockIdx.y];
   if (level >= S.sub_level) return;
@@ -676,7 +717,8 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_count(ListBufs LB, TreeDe
   uint32_t base = s + q * LVL_TILE;
   if (c == 0) return;
   for (int d = 0; d < 3; ++d) {
-    if (d == sd) continue;  // the split list is not partitioned: no count (uniform)
+    // the split list is not partitioned, the lazy one not kept (uniform)
+    if (d == sd || d == lazy) continue;
     // list entries, then their flag words, each as a batch of independent
     // loads (indices clamped into the segment); items past e count 0
     const uint32_t *ld = LB.in(t.lmask[h], d);
@@ -739,7 +781,7 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(ListBufs LB,
                                                                uint32_t level,
                                                                uint32_t T,
                                                                const uint32_t *__restrict__ bits,
-                                                               const uint32_t *cnt) {
+                                                               const uint32_t *cnt, int lazy) {
   __shared__ uint32_t sh[LVL_THREADS / 64];
   const SysDesc S = sys[blockIdx.y];
   if (level >= S.sub_level) return;
@@ -755,7 +797,9 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(ListBufs LB,
   uint32_t base = s + q * LVL_TILE + threadIdx.x * LVL_ITEMS;
   const uint32_t m = t.lmask[h];
   for (int d = 0; d < 3; ++d) {
-    if (d == sd) continue;  // already partitioned at the median: stays in place
+    // the split list is already partitioned at the median and stays in
+    // place; the lazy list is not kept
+    if (d == sd || d == lazy) continue;
     const uint32_t *ld = LB.in(m, d);
     uint32_t *od = LB.out(m, d);
     uint32_t id[LVL_ITEMS];
@@ -804,6 +848,7 @@ struct SubArgs {
   float4 *pos32;
   int leaf;
   double theta2;
+  int lazy;  // a dimension whose list the device-wide levels did not keep (3: none)
 };
 
 // Heap index (1-based) of the node after the subtree of heap slot h (0-based).
@@ -932,7 +977,11 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   if (k >= (1u << level)) return;
   const TreeDev t = tree_at(a.t, S.node_off);
   const uint32_t m0 = t.lmask[(1u << level) - 1 + k];
-  const uint32_t *L0 = a.LB.in(m0, 0), *L1 = a.LB.in(m0, 1), *L2 = a.LB.in(m0, 2);
+  // the segment's ids come from list g0; g1 (and 2 when no dimension is
+  // lazy) are read as local indices, the lazy one is sorted here
+  const int lz = a.lazy;
+  const int g0 = lz == 0 ? 1 : 0, g1 = lz == 1 || lz == 0 ? 2 : 1;
+  const uint32_t *L0 = a.LB.in(m0, g0), *L1 = a.LB.in(m0, g1), *L2 = a.LB.in(m0, 2);
   const uint32_t h0 = (1u << level) - 1 + k;
   // s0l: first body of the segment within its system; s0: in the body
   // arrays (tree order); sl: in the sorted lists
@@ -946,16 +995,52 @@ __global__ __launch_bounds__(SUB_THREADS) void k_subtree(SubArgs a) {
   for (uint32_t i = tid; i < n0; i += NT) {
     uint32_t id = L0[sl + i];
     ids[i] = id;
-    lst[0][0][i] = (uint16_t)i;
+    lst[0][g0][i] = (uint16_t)i;
     a.loc[id] = i;
   }
   __threadfence_block();
   __syncthreads();
   for (uint32_t i = tid; i < n0; i += NT) {
-    lst[0][1][i] = (uint16_t)a.loc[L1[sl + i]];
-    lst[0][2][i] = (uint16_t)a.loc[L2[sl + i]];
+    lst[0][g1][i] = (uint16_t)a.loc[L1[sl + i]];
+    if (lz == 3) lst[0][2][i] = (uint16_t)a.loc[L2[sl + i]];
   }
   __syncthreads();
+  if (lz < 3) {
+    // the lazy dimension's order: a bitonic sort of (ordered coordinate key,
+    // working id) pairs, the order of the presorted lists; the keys use the
+    // LDS of the second list buffer, flags and sdarr (not live yet), the
+    // local indices end in lst[0][lz]
+    uint64_t *kz = reinterpret_cast<uint64_t *>(smem + 10 * SUB_MAX);
+    uint16_t *iz = lst[0][lz];
+    static_assert(18 * SUB_MAX - 10 * SUB_MAX >= 8 * SUB_MAX, "lazy-sort keys need 8 SUB_MAX bytes");
+    uint32_t M = 2;
+    while (M < n0) M <<= 1;
+    const double *zc = a.P.c[lz];
+    for (uint32_t i = tid; i < M; i += NT) {
+      kz[i] = i < n0 ? ordkey(zc[ids[i]]) : ~0ull;  // padding sorts last
+      iz[i] = (uint16_t)(i < n0 ? i : 0xFFFFu);
+    }
+    __syncthreads();
+    for (uint32_t size = 2; size <= M; size <<= 1) {
+      for (uint32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (uint32_t p = tid; p < M / 2; p += NT) {
+          const uint32_t i = 2 * p - (p & (stride - 1)), j = i + stride;
+          const bool up = (i & size) == 0;
+          const uint64_t ki = kz[i], kj = kz[j];
+          const uint16_t li = iz[i], lj = iz[j];
+          // equal keys (never two paddings' ids) go by working id
+          const bool gt = ki > kj || (ki == kj && ki != ~0ull && ids[li] > ids[lj]);
+          if (gt == up) {
+            kz[i] = kj;
+            kz[j] = ki;
+            iz[i] = lj;
+            iz[j] = li;
+          }
+        }
+        __syncthreads();
+      }
+    }
+  }
 
   uint32_t st[IT], ct[IT], cc[IT], hl[IT];
   bool alive[IT];
@@ -1338,10 +1423,15 @@ hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t
 
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
-                        double theta, hipStream_t s, bool range_ready) {
+                        double theta, hipStream_t s, bool range_ready, int lazy, bool report) {
   const double *coord[3] = {in.x, in.y, in.z};
   const unsigned ny = (unsigned)f.nsys;
   hipError_t e;
+  if (f.nsys > 1) lazy = 3;  // a forest keeps every list
+  if (report) {
+    e = hipMemsetAsync(b.viol, 0, sizeof(uint32_t), s);
+    if (e != hipSuccess) return e;
+  }
   // a single system sorts its n bodies; a forest sorts every body position
   // by coordinate, then stably by system, which leaves each system's bodies
   // in coordinate order at [list_off, list_off + n) (padding last)
@@ -1360,6 +1450,7 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   if (e != hipSuccess) return e;
   for (int d = 0; d < 3; ++d) {
     const hipStream_t sd = d == 0 ? s : b.side[d - 1];
+    if (d == lazy) continue;  // sorted per segment by the subtree builder
     uint32_t *key32 = b.keys_in[d];
     uint32_t *key32_out = b.keys_out[d];
     k_make_keys32<<<(nsort + 255) / 256, 256, 0, sd>>>(
@@ -1397,17 +1488,24 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   for (int lev = 0; lev < f.max_sub_level; ++lev) {
     const uint32_t nseg = 1u << lev;
     const uint32_t T = level_tiles(f, lev);
-    k_level_split<<<dim3(nseg, ny), SPLIT_THREADS, 0, s>>>(LB, P, t, f.sys, in.oid, lev);
+    k_level_split<<<dim3(nseg, ny), SPLIT_THREADS, 0, s>>>(LB, P, t, f.sys, in.oid, lev,
+                                                           b.range, lazy,
+                                                           report ? b.viol : nullptr);
     k_level_flags<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.flags);
     k_level_pack<<<(b.flag_words + 255) / 256, 256, 0, s>>>(b.flags, b.bits, b.flag_words);
     k_level_count<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.bits,
-                                                             b.cnt);
+                                                             b.cnt, lazy);
     if (T > SCAN_SMALL_T)
       k_level_scan<1024><<<dim3(3 * nseg, ny), 1024, 0, s>>>(b.cnt, T, f.sys, lev);
     else
       k_level_scan<64><<<dim3(3 * nseg, ny), 64, 0, s>>>(b.cnt, T, f.sys, lev);
     k_level_scatter<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.bits,
-                                                               b.cnt);
+                                                               b.cnt, lazy);
+  }
+  if (report) {
+    e = hipMemcpyAsync(b.viol_host, b.viol, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipEventRecord(b.ev_viol, s);
+    if (e != hipSuccess) return e;
   }
   SubArgs a;
   a.LB = LB;
@@ -1423,6 +1521,7 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   a.pos32 = pos32;
   a.leaf = leaf;
   a.theta2 = theta * theta;
+  a.lazy = lazy;
   k_subtree<<<dim3(1u << f.max_sub_level, ny), SUB_THREADS, 0, s>>>(a);
   if (f.max_sub_level > 0) k_moments_top<<<ny, 1024, 0, s>>>(t, f.sys, theta * theta);
   return hipGetLastError();

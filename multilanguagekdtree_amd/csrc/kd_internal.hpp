@@ -231,6 +231,13 @@ struct BuildBufs {
   uint32_t *loc;     // working id -> local index inside a subtree block
   uint32_t *cnt;     // per-tile left counts of the device-wide levels
   uint32_t *perm;    // tree position -> working id
+  // lazy dimension (single systems): bit d of *viol is set when dimension d
+  // could be the split dimension of a device-wide segment, judged by its
+  // global extent; viol_host (pinned) receives it after the levels, ev_viol
+  // marks that copy
+  uint32_t *viol = nullptr;
+  uint32_t *viol_host = nullptr;
+  hipEvent_t ev_viol = nullptr;
 };
 
 Shape make_shape(int64_t n, int leaf);
@@ -241,9 +248,17 @@ size_t level_cnt_entries(const Forest &f);
 hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t s);
 // range_ready: b.range already holds the range of `in` (computed by the
 // kick-drift that produced it), so the build skips its own range pass.
+// lazy (single systems, 0-2; 3 for none): that dimension is neither sorted
+// nor partitioned at the device-wide levels, whose splits take its global
+// extent as the bound of every segment's, and the subtree builder sorts it
+// per segment; the tree is exact unless bit `lazy` of *b.viol is set.
+// report: clear *b.viol before the levels, then copy it to b.viol_host and
+// record b.ev_viol after them (a full build reports every dimension that a
+// lazy build could not have skipped).
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
-                        double theta, hipStream_t s, bool range_ready = false);
+                        double theta, hipStream_t s, bool range_ready = false, int lazy = 3,
+                        bool report = false);
 hipError_t launch_traverse(int precision, bool count, int variant, const TreeDev &t,
                            const double4 *pos64, const float4 *pos32, uint32_t n,
                            double theta, int leaf, double *ax, double *ay, double *az,

@@ -159,6 +159,11 @@ class Simulation:
     def set_tuning(self, key: str, value: int):
         check(self._L.kd_set_tuning(self._ctx, key.encode(), int(value)))
 
+    def get_tuning(self, key: str) -> int:
+        out = np.zeros(1, np.int64)
+        check(self._L.kd_get_tuning(self._ctx, key.encode(), iptr(out)))
+        return int(out[0])
+
     def timings(self) -> dict:
         """Summed HIP-event times (ms) of kd_step phases since the last reset."""
         out = np.zeros(4)

@@ -1,0 +1,100 @@
+"""The build's lazy dimension: in a thin system one dimension can never be
+the split dimension of a device-wide segment (its global extent is below the
+segment's other extents), so a build may skip its sort and its partitions at
+those levels and order it per segment in the subtree builder instead.  Trees
+must be the oracle's bit for bit whichever path built them
+(Parallel/RustVersion/src/array_kd_tree.rs:63-130): thin disks in each
+coordinate plane (so each dimension is the lazy one once), the path turned
+off, and a build whose check fails (a thick system after a thin one in the
+same context), which must be repeated in full."""
+import numpy as np
+import pytest
+
+from helpers import compare_trees, pscale
+from multilanguagekdtree_amd import _lib
+from multilanguagekdtree_amd import kdtree as K
+
+pytestmark = pytest.mark.gpu
+
+
+def disk(n, thin, seed):
+    """A ring-like disk whose thin axis is `thin` (0, 1 or 2)."""
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(1.0, 2.0, n)
+    phi = rng.uniform(0.0, 2 * np.pi, n)
+    a, b, c = r * np.cos(phi), r * np.sin(phi), rng.normal(0.0, 1e-3, n)
+    axes = [a, b]
+    axes.insert(thin, c)
+    p = np.stack(axes, 1)
+    return dict(x=p[:, 0].copy(), y=p[:, 1].copy(), z=p[:, 2].copy(), vx=np.zeros(n),
+                vy=np.zeros(n), vz=np.zeros(n), m=np.full(n, 1e-14))
+
+
+def cube(n, seed):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-1.0, 1.0, (n, 3))
+    return dict(x=p[:, 0].copy(), y=p[:, 1].copy(), z=p[:, 2].copy(), vx=np.zeros(n),
+                vy=np.zeros(n), vz=np.zeros(n), m=np.full(n, 1.0 / n))
+
+
+N = 300_000  # eight device-wide levels above the subtree builder
+
+
+@pytest.mark.parametrize("thin", [2, 0, 1])
+def test_thin_disk_builds_lazily_and_exactly(orc, thin):
+    b = disk(N, thin, seed=80 + thin)
+    r = orc.build_tree(b)
+    with K.Simulation(N) as s:
+        s.upload(b)
+        s.build()  # a full build, which finds the thin dimension safe to skip
+        assert s.get_tuning("lazy_next") == thin
+        compare_trees(s.export_tree(K.KD_LAYOUT_COMPACT), r, pscale(b))
+        s.build()  # skips it
+        assert s.get_tuning("lazy_builds") == 1 and s.get_tuning("lazy_redos") == 0
+        compare_trees(s.export_tree(K.KD_LAYOUT_COMPACT), r, pscale(b))
+        compare_trees(s.export_tree(K.KD_LAYOUT_PADDED), orc.build_tree(b, 7, K.KD_LAYOUT_PADDED),
+                      pscale(b))
+
+
+def test_lazy_and_full_trees_and_forces_agree():
+    b = K.ring(N, seed=84)
+    out = []
+    for on in (1, 0):
+        with K.Simulation(N) as s:
+            s.set_tuning("lazy_dimension", on)
+            s.upload(b)
+            s.step(3)
+            st = s.download()
+            out.append((st, s.get_tuning("lazy_builds")))
+    (lazy_state, lazy_builds), (full_state, full_builds) = out
+    assert lazy_builds >= 2 and full_builds == 0
+    for k in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(lazy_state[k], full_state[k], err_msg=k)
+
+
+def test_failed_check_builds_again_in_full(orc):
+    thin, thick = disk(N, 2, seed=85), cube(N, seed=86)
+    with K.Simulation(N) as s:
+        s.upload(thin)
+        s.build()
+        assert s.get_tuning("lazy_next") == 2
+        # the same context now holds a cube: the lazy build's own check finds
+        # that z can be split at a device-wide level, and the build is repeated
+        s.upload(thick)
+        s.build()
+        assert s.get_tuning("lazy_builds") == 1 and s.get_tuning("lazy_redos") == 1
+        compare_trees(s.export_tree(K.KD_LAYOUT_COMPACT), orc.build_tree(thick), pscale(thick))
+        # the full build found no dimension to skip; a thick system stays full
+        assert s.get_tuning("lazy_next") == 3
+        s.build()
+        assert s.get_tuning("lazy_builds") == 1
+        compare_trees(s.export_tree(K.KD_LAYOUT_COMPACT), orc.build_tree(thick), pscale(thick))
+
+
+def test_tuning_keys():
+    with K.Simulation(1000) as s:
+        assert s.get_tuning("lazy_dimension") == 1
+        s.set_tuning("lazy_dimension", 0)
+        assert s.get_tuning("lazy_dimension") == 0 and s.get_tuning("lazy_next") == 3
+        with pytest.raises(_lib.KdError):
+            s.get_tuning("no_such_key")
