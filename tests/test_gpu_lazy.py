@@ -98,3 +98,20 @@ def test_tuning_keys():
         assert s.get_tuning("lazy_dimension") == 0 and s.get_tuning("lazy_next") == 3
         with pytest.raises(_lib.KdError):
             s.get_tuning("no_such_key")
+
+
+@pytest.mark.parametrize("n", [1, 9, 100, 1000, 2047, 2048, 2049, 5000, 40_000])
+def test_small_and_ragged_segments(orc, n):
+    """Segment sizes that leave the per-segment sort's last block ragged
+    (every n0 from a single body to a full segment), thin disks whose z is
+    drawn from three values so that the sort's ties are broken by working
+    id, built twice in one context (the second build sorts z per segment)."""
+    b = disk(n, 2, seed=90 + n % 7)
+    b["z"] = np.round(b["z"] * 1e3) * 1e-3
+    r = orc.build_tree(b)
+    with K.Simulation(n) as s:
+        s.upload(b)
+        s.build()
+        s.build()
+        assert s.get_tuning("lazy_builds") == 1, n
+        compare_trees(s.export_tree(K.KD_LAYOUT_COMPACT), r, pscale(b))
