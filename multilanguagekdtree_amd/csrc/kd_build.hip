@@ -953,52 +953,77 @@ __device__ __forceinline__ bool sub_node(uint32_t c, int lam, uint32_t n0, int l
   return true;
 }
 
-// The lazy dimension's order inside a segment: a sort of (ordered coordinate
-// key, working id) pairs, the order of the presorted lists.  A bitonic network
-// in the form whose comparators all sort upward: the merge of two sorted
-// blocks into one of `size` positions first compares mirror positions x and
-// x ^ (size - 1), then x and x + stride for strides size/4 .. 1.  Positions
-// at or past n0 act as +inf: they are never read or written, and every
-// comparator whose upper position is one of them is skipped.  A pass runs up
-// to LZ_G consecutive stages of one merge in registers, 2^g positions a
-// thread, so the keys cross LDS once per LZ_G stages, not once per stage.
-// At 10 M ring bodies (segments of ~1220) a build takes 3.18 ms with 2
-// stages a pass, against 3.27 ms with 3 (half the threads idle) and 3.45 ms
-// for one LDS round trip per stage over every position of the padded block.
+// The lazy dimension's order inside a segment: the order of the presorted
+// lists, (coordinate, working id).  Sorted by a bitonic network in the form
+// whose comparators all sort upward: the merge of two sorted blocks into one
+// of `size` positions first compares mirror positions x and x ^ (size - 1),
+// then x and x + stride for strides size/4 .. 1.  Positions at or past n0
+// act as +inf: they are never read or written, and every comparator whose
+// upper position is one of them is skipped.  A pass runs up to LZ_G
+// consecutive stages of one merge in registers, 2^g positions a thread, so
+// the values cross LDS once per LZ_G stages, not once per stage.  At 10 M
+// ring bodies (segments of ~1220) a build takes 3.07 ms with 2 stages a pass
+// on 32-bit values (3.09 ms with 3 stages, half the threads idle; 3.17 ms on
+// 64-bit keys; 3.45 ms for one LDS round trip per stage over every position
+// of the padded block).
 #ifndef KD_LZ_G
 #define KD_LZ_G 2
 #endif
 constexpr int LZ_G = KD_LZ_G;
 
-// one comparator; equal keys (equal coordinates) go by working id
-__device__ __forceinline__ void lz_cmp(uint64_t &ka, uint16_t &la, uint64_t &kb, uint16_t &lb,
-                                       const uint32_t *ids) {
-  if (ka > kb || (ka == kb && ids[la] > ids[lb])) {
-    const uint64_t k = ka;
-    const uint16_t l = la;
-    ka = kb;
-    la = lb;
-    kb = k;
-    lb = l;
+// The values the network sorts, in two forms.  Packed: a 21-bit fixed-point
+// key of the coordinate over the segment's own range and the 11-bit local
+// index in one word (4 bytes a value, one compare a comparator); bodies whose
+// keys collide are put in exact order afterwards.  Wide: the 64-bit ordered
+// key of the coordinate and the local index, equal keys by working id (exact
+// by itself; the form a segment falls back to when keys collide in runs
+// longer than LZ_RUN).
+struct LzPacked {
+  uint32_t *k;
+  using V = uint32_t;
+  __device__ V get(uint32_t p) const { return k[p]; }
+  __device__ void put(uint32_t p, V v) const { k[p] = v; }
+  __device__ void order(V &a, V &b) const {
+    const V lo = a < b ? a : b, hi = a < b ? b : a;
+    a = lo;
+    b = hi;
   }
-}
+};
+struct LzWide {
+  uint64_t *k;
+  uint16_t *l;
+  const uint32_t *ids;
+  struct V {
+    uint64_t k;
+    uint16_t l;
+  };
+  __device__ V get(uint32_t p) const { return V{k[p], l[p]}; }
+  __device__ void put(uint32_t p, const V &v) const {
+    k[p] = v.k;
+    l[p] = v.l;
+  }
+  __device__ void order(V &a, V &b) const {
+    if (a.k > b.k || (a.k == b.k && ids[a.l] > ids[b.l])) {
+      const V t = a;
+      a = b;
+      b = t;
+    }
+  }
+};
+constexpr uint32_t LZ_KEY_BITS = 21, LZ_KEY_MAX = (1u << LZ_KEY_BITS) - 1, LZ_RUN = 16;
+static_assert(SUB_MAX <= (1 << (32 - LZ_KEY_BITS)), "local indices fill the packed low bits");
 
-// Merges of sizes 2 .. 2^G over runs of 2^G consecutive positions, with the
-// keys read from the coordinates.
-template <int NT, int G>
-__device__ __forceinline__ void lz_first(uint64_t *kz, uint16_t *iz, const uint32_t *ids,
-                                         const double *zc, uint32_t n0, uint32_t M, int tid) {
+// Merges of sizes 2 .. 2^G over runs of 2^G consecutive positions.
+template <int NT, int G, class Pol>
+__device__ __forceinline__ void lz_first(const Pol &pol, uint32_t n0, uint32_t M, int tid) {
   constexpr int E = 1 << G;
   for (uint32_t t = tid; t < (M >> G); t += NT) {
     const uint32_t a = t << G;
     if (a >= n0) break;
-    uint64_t k[E];
-    uint16_t l[E];
+    typename Pol::V v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      k[e] = a + e < n0 ? ordkey(zc[ids[a + e]]) : 0;
-      l[e] = (uint16_t)(a + e);
-    }
+    for (int e = 0; e < E; ++e)
+      if (a + e < n0) v[e] = pol.get(a + e);
 #pragma unroll
     for (int q = 1; q <= G; ++q) {
 #pragma unroll
@@ -1007,29 +1032,26 @@ __device__ __forceinline__ void lz_first(uint64_t *kz, uint16_t *iz, const uint3
         for (int e = 0; e < E; ++e) {
           if (e & (1 << r)) continue;
           const int f = r == q - 1 ? e ^ ((1 << q) - 1) : e + (1 << r);
-          if (a + f < n0) lz_cmp(k[e], l[e], k[f], l[f], ids);
+          if (a + f < n0) pol.order(v[e], v[f]);
         }
       }
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      if (a + e < n0) {
-        kz[a + e] = k[e];
-        iz[a + e] = l[e];
-      }
-    }
+    for (int e = 0; e < E; ++e)
+      if (a + e < n0) pol.put(a + e, v[e]);
   }
 }
 
 // G stages of the merge into blocks of `size`, with strides 2^ls ..
-// 2^(ls-G+1); MIRROR: the first is the mirror stage (2^ls = size / 2).  Set t
+// 2^(ls-G+1), less the first S of them; MIRROR: the first is the mirror
+// stage (2^ls = size / 2).  Set t
 // holds the positions a + the sums of subsets of those strides, a = t with
 // zero bits inserted at the strides' bit positions; with MIRROR the upper
 // half of the set is the mirror image of the lower half, listed so that
 // positions still increase with the set index.
-template <int NT, int G, bool MIRROR>
-__device__ __forceinline__ void lz_pass(uint64_t *kz, uint16_t *iz, const uint32_t *ids,
-                                        uint32_t n0, uint32_t M, uint32_t size, int ls, int tid) {
+template <int NT, int G, bool MIRROR, int S, class Pol>
+__device__ __forceinline__ void lz_pass(const Pol &pol, uint32_t n0, uint32_t M, uint32_t size,
+                                        int ls, int tid) {
   constexpr int E = 1 << G;
   const int lo = ls - G + 1;
   for (uint32_t t = tid; t < (M >> G); t += NT) {
@@ -1047,63 +1069,181 @@ __device__ __forceinline__ void lz_pass(uint64_t *kz, uint16_t *iz, const uint32
       }
       pos[e] = (MIRROR && (e >> (G - 1))) ? (a + full - off) ^ (size - 1) : a + off;
     }
-    uint64_t k[E];
-    uint16_t l[E];
+    typename Pol::V v[E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      k[e] = pos[e] < n0 ? kz[pos[e]] : 0;
-      l[e] = pos[e] < n0 ? iz[pos[e]] : 0;
-    }
+    for (int e = 0; e < E; ++e)
+      if (pos[e] < n0) v[e] = pol.get(pos[e]);
 #pragma unroll
-    for (int j = 0; j < G; ++j) {
+    for (int j = S; j < G; ++j) {
       const int bit = 1 << (G - 1 - j);
 #pragma unroll
       for (int e = 0; e < E; ++e) {
         if (e & bit) continue;
         const int f = (MIRROR && j == 0) ? e ^ (E - 1) : e | bit;
-        if (pos[f] < n0) lz_cmp(k[e], l[e], k[f], l[f], ids);
+        if (pos[f] < n0) pol.order(v[e], v[f]);
       }
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      if (pos[e] < n0) {
-        kz[pos[e]] = k[e];
-        iz[pos[e]] = l[e];
-      }
-    }
+    for (int e = 0; e < E; ++e)
+      if (pos[e] < n0) pol.put(pos[e], v[e]);
   }
 }
 
-template <int NT, bool MIRROR>
-__device__ __forceinline__ void lz_pass_g(int g, uint64_t *kz, uint16_t *iz, const uint32_t *ids,
-                                          uint32_t n0, uint32_t M, uint32_t size, int ls, int tid) {
-  static_assert(LZ_G >= 1 && LZ_G <= 4, "lazy-sort passes of 1 to 4 stages");
-  if (g >= 4 && LZ_G >= 4) lz_pass<NT, (LZ_G >= 4 ? 4 : 1), MIRROR>(kz, iz, ids, n0, M, size, ls, tid);
-  else if (g == 3 && LZ_G >= 3) lz_pass<NT, (LZ_G >= 3 ? 3 : 1), MIRROR>(kz, iz, ids, n0, M, size, ls, tid);
-  else if (g == 2 && LZ_G >= 2) lz_pass<NT, 2, MIRROR>(kz, iz, ids, n0, M, size, ls, tid);
-  else lz_pass<NT, 1, MIRROR>(kz, iz, ids, n0, M, size, ls, tid);
+// LDS writes of a wavefront made visible to its own lanes.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The whole sort of n0 <= SUB_MAX positions; ends with a barrier.
-template <int NT>
-__device__ void lz_sort(uint64_t *kz, uint16_t *iz, const uint32_t *ids, const double *zc,
-                        uint32_t n0, int tid) {
-  int lm = LZ_G;
+// The network over the n0 <= SUB_MAX values already in LDS; ends with a
+// barrier.  Every pass gives thread t one set of 2^G positions (a merge's
+// last stages that do not fill a pass run as the last pass shape with its
+// first stages skipped), so a pass whose largest stride is below 2^(6+G)
+// keeps wavefront w within positions [64 w 2^G, 64 (w+1) 2^G): between two
+// such passes the wavefront only waits for its own LDS writes.
+template <int NT, int G, class Pol>
+__device__ void lz_sort(const Pol &pol, uint32_t n0, int tid) {
+  static_assert((SUB_MAX >> G) <= NT, "one set of positions a thread");
+  static_assert(G >= 2 && G <= 3, "passes of 2 or 3 stages");
+  constexpr int WL = 6 + G;
+  int lm = G;
   while ((1u << lm) < n0) ++lm;
   const uint32_t M = 1u << lm;
-  lz_first<NT, LZ_G>(kz, iz, ids, zc, n0, M, tid);
-  __syncthreads();
-  for (int m = LZ_G + 1; m <= lm; ++m) {
+  lz_first<NT, G>(pol, n0, M, tid);
+  int prev = G - 1;  // the largest stride of the last pass, as a power of 2
+  auto sync = [&](int next) {
+    if (prev + 1 <= WL && next + 1 <= WL) wave_sync();
+    else __syncthreads();
+    prev = next;
+  };
+  for (int m = G + 1; m <= lm; ++m) {
     const uint32_t size = 1u << m;
-    // m > LZ_G stages: the mirror stage and LZ_G - 1 strides, then the rest
-    lz_pass<NT, LZ_G, true>(kz, iz, ids, n0, M, size, m - 1, tid);
-    __syncthreads();
-    for (int ls = m - 1 - LZ_G; ls >= 0;) {
-      const int g = ls + 1 < LZ_G ? ls + 1 : LZ_G;
-      lz_pass_g<NT, false>(g, kz, iz, ids, n0, M, size, ls, tid);
-      __syncthreads();
-      ls -= g;
+    // m > G stages: the mirror stage and G - 1 strides, then the rest
+    sync(m - 1);
+    lz_pass<NT, G, true, 0>(pol, n0, M, size, m - 1, tid);
+    for (int ls = m - 1 - G; ls >= 0; ls -= G) {
+      if (ls + 1 >= G) {
+        sync(ls);
+        lz_pass<NT, G, false, 0>(pol, n0, M, size, ls, tid);
+      } else {
+        // the last ls + 1 < G stages: strides 2^ls .. 1
+        sync(G - 1);
+        if (G - 1 - ls == 1) lz_pass<NT, G, false, 1>(pol, n0, M, size, G - 1, tid);
+        else lz_pass<NT, G, false, (G >= 3 ? 2 : 1)>(pol, n0, M, size, G - 1, tid);
+      }
     }
+  }
+  __syncthreads();
+}
+
+// The lazy dimension's order of a segment: local indices into iz.  buf: 8
+// SUB_MAX bytes of LDS; shd: 2 NT / 64 doubles; flagp: one word.  The packed
+// form first; a run of more than LZ_RUN colliding keys (a segment whose
+// coordinates are all equal, or nearly) sends the segment to the wide form.
+template <int NT>
+__device__ void lz_order(unsigned char *buf, uint16_t *iz, const uint32_t *ids, const double *zc,
+                         uint32_t n0, int tid, double *shd, uint32_t *flagp) {
+  constexpr int IT = SUB_MAX / NT;
+  constexpr int NW = NT / 64;
+  double z[IT];
+  double lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+  for (int q = 0; q < IT; ++q) {
+    const uint32_t i = tid + q * NT;
+    z[q] = i < n0 ? zc[ids[i]] : 0.0;
+    if (i < n0) {
+      lo = fmin(lo, z[q]);
+      hi = fmax(hi, z[q]);
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = fmin(lo, __shfl_xor(lo, o, 64));
+    hi = fmax(hi, __shfl_xor(hi, o, 64));
+  }
+  if ((tid & 63) == 0) {
+    shd[tid >> 6] = lo;
+    shd[NW + (tid >> 6)] = hi;
+  }
+  if (tid == 0) *flagp = 0;
+  __syncthreads();
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    lo = fmin(lo, shd[w]);
+    hi = fmax(hi, shd[NW + w]);
+  }
+  // all coordinates equal (a planar system) leaves the order to the working
+  // ids, which the wide form gives directly
+  if (hi > lo) {
+    // a monotone map of the coordinate, so distinct keys are in coordinate
+    // order and only equal keys need the exact comparison
+    const double scale = (double)LZ_KEY_MAX / (hi - lo);
+    uint32_t *kp = reinterpret_cast<uint32_t *>(buf);
+#pragma unroll
+    for (int q = 0; q < IT; ++q) {
+      const uint32_t i = tid + q * NT;
+      if (i < n0) {
+        const uint32_t key = (uint32_t)fmin(fmax((z[q] - lo) * scale, 0.0), (double)LZ_KEY_MAX);
+        kp[i] = (key << (32 - LZ_KEY_BITS)) | i;
+      }
+    }
+    __syncthreads();
+    lz_sort<NT, LZ_G>(LzPacked{kp}, n0, tid);
+    constexpr uint32_t IMASK = (1u << (32 - LZ_KEY_BITS)) - 1;
+#pragma unroll
+    for (int q = 0; q < IT; ++q) {
+      const uint32_t i = tid + q * NT;
+      if (i < n0) iz[i] = (uint16_t)(kp[i] & IMASK);
+    }
+    __syncthreads();
+    // runs of equal keys, each ordered by the thread at its start by
+    // (coordinate, working id)
+#pragma unroll
+    for (int q = 0; q < IT; ++q) {
+      const uint32_t i = tid + q * NT;
+      if (i + 1 >= n0) continue;
+      const uint32_t key = kp[i] >> (32 - LZ_KEY_BITS);
+      if ((kp[i + 1] >> (32 - LZ_KEY_BITS)) != key) continue;
+      if (i > 0 && (kp[i - 1] >> (32 - LZ_KEY_BITS)) == key) continue;
+      uint32_t j = i + 1;
+      while (j < n0 && j - i <= LZ_RUN && (kp[j] >> (32 - LZ_KEY_BITS)) == key) ++j;
+      if (j - i > LZ_RUN) {
+        *flagp = 1;
+        continue;
+      }
+      for (uint32_t x = i + 1; x < j; ++x) {
+        const uint16_t v = iz[x];
+        const uint32_t idv = ids[v];
+        const double zv = zc[idv];
+        uint32_t y = x;
+        while (y > i) {
+          const uint16_t u = iz[y - 1];
+          const uint32_t idu = ids[u];
+          const double zu = zc[idu];
+          if (zu < zv || (zu == zv && idu < idv)) break;
+          iz[y] = u;
+          --y;
+        }
+        iz[y] = v;
+      }
+    }
+  } else if (tid == 0) {
+    *flagp = 1;
+  }
+  __syncthreads();
+  if (*flagp) {
+    uint64_t *kz = reinterpret_cast<uint64_t *>(buf);
+#pragma unroll
+    for (int q = 0; q < IT; ++q) {
+      const uint32_t i = tid + q * NT;
+      if (i < n0) {
+        kz[i] = ordkey(z[q]);
+        iz[i] = (uint16_t)i;
+      }
+    }
+    __syncthreads();
+    lz_sort<NT, 2>(LzWide{kz, iz, ids}, n0, tid);
   }
 }
 
@@ -1130,6 +1270,9 @@ k_subtree(SubArgs a) {
   // per level (by parity): some segment of the level splits, so bodies are
   // still alive below it
   __shared__ uint32_t splits[2];
+  // the lazy dimension's sort: block range of the coordinate, fallback flag
+  __shared__ double shd[2 * NT / 64];
+  __shared__ uint32_t lzflag;
 
   const SysDesc S = a.sys[blockIdx.y];
   const uint32_t k = blockIdx.x;
@@ -1170,8 +1313,7 @@ k_subtree(SubArgs a) {
     // buffer, flags and sdarr (not live yet), the local indices end in
     // lst[0][lz]
     static_assert(18 * SUB_MAX - 10 * SUB_MAX >= 8 * SUB_MAX, "lazy-sort keys need 8 SUB_MAX bytes");
-    lz_sort<NT>(reinterpret_cast<uint64_t *>(smem + 10 * SUB_MAX), lst[0][lz], ids, a.P.c[lz], n0,
-                tid);
+    lz_order<NT>(smem + 10 * SUB_MAX, lst[0][lz], ids, a.P.c[lz], n0, tid, shd, &lzflag);
   }
 
   uint32_t st[IT], ct[IT], cc[IT], hl[IT];
@@ -1503,10 +1645,16 @@ SysDesc make_sys(const Shape &sh, uint32_t body_off, uint32_t node_off, uint32_t
 // 10 items per tile.  On gfx950 this sorts 10 M pairs in ~320 us, against
 // ~400 us for rocprim's default tile of 1024 x 16 (wider digits, 10 or 11
 // bits in three passes, measured slower).
+#ifndef KD_SORT_BITS
+#define KD_SORT_BITS 8
+#endif
+#ifndef KD_SORT_ITEMS
+#define KD_SORT_ITEMS 10
+#endif
 using SortCfg = rocprim::radix_sort_config<
     rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 10>,
-                                        rocprim::kernel_config<1024, 10>, 8,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, KD_SORT_ITEMS>,
+                                        rocprim::kernel_config<1024, KD_SORT_ITEMS>, KD_SORT_BITS,
                                         rocprim::block_radix_rank_algorithm::match>>;
 
 static int bits_for(uint32_t v) {

@@ -100,14 +100,36 @@ def test_tuning_keys():
             s.get_tuning("no_such_key")
 
 
+def thin_z(n, kind, seed):
+    """Thin disks whose z exercises each path of the per-segment sort: `dups`
+    gives exact duplicates and values one ulp apart (short runs of equal
+    fixed-point keys, put in exact order one by one), `few` draws z from
+    nine values (long runs: the segment falls back to the 64-bit keys, ties
+    by working id), `planar` has z = 0 exactly (all keys equal: order by
+    working id)."""
+    b = disk(n, 2, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    z = b["z"]
+    if kind == "dups":
+        k = n // 5
+        src, dst = rng.integers(0, n, k), rng.integers(0, n, k)
+        z[dst] = z[src]
+        src, dst = rng.integers(0, n, k // 2), rng.integers(0, n, k // 2)
+        z[dst] = np.nextafter(z[src], np.inf)
+    elif kind == "few":
+        z[:] = np.round(z * 1e3) * 1e-3
+    else:
+        z[:] = 0.0
+    return b
+
+
+@pytest.mark.parametrize("kind", ["dups", "few", "planar"])
 @pytest.mark.parametrize("n", [1, 9, 100, 1000, 2047, 2048, 2049, 5000, 40_000])
-def test_small_and_ragged_segments(orc, n):
+def test_small_and_ragged_segments(orc, n, kind):
     """Segment sizes that leave the per-segment sort's last block ragged
-    (every n0 from a single body to a full segment), thin disks whose z is
-    drawn from three values so that the sort's ties are broken by working
-    id, built twice in one context (the second build sorts z per segment)."""
-    b = disk(n, 2, seed=90 + n % 7)
-    b["z"] = np.round(b["z"] * 1e3) * 1e-3
+    (every n0 from a single body to a full segment), built twice in one
+    context (the second build sorts z per segment)."""
+    b = thin_z(n, kind, seed=90 + n % 7)
     r = orc.build_tree(b)
     with K.Simulation(n) as s:
         s.upload(b)
