@@ -170,7 +170,11 @@ int kd_counters(kd_ctx *ctx, int64_t out[5]);
  * dimension that the previous full build found could not be split at those
  * levels (a thin system's third dimension); every build checks that this
  * holds and builds again in full when it does not, so trees are the same
- * either way.  Unknown keys return KD_ERR_ARG. */
+ * either way; "fused_kick_drift" 0/1 (default 1) -- kd_step's walk applies
+ * the kick-drift to each body as it finishes it, with no acceleration arrays
+ * and no separate pass (steps with params.count_interactions set keep the
+ * separate pass), bit-identical either way.  Unknown keys return
+ * KD_ERR_ARG. */
 int kd_set_tuning(kd_ctx *ctx, const char *key, int64_t value);
 
 /* Reads a knob of kd_set_tuning, or the lazy-dimension state: "lazy_next"
@@ -181,7 +185,8 @@ int kd_get_tuning(kd_ctx *ctx, const char *key, int64_t *value);
 
 /* With params.profile set, kd_step accumulates per phase the HIP-event time
  * on the context's stream: out = {build ms, traversal ms, kick-drift ms,
- * steps timed}.  kd_reset_timings zeroes the sums.  No counterpart in the
+ * steps timed} (with the fused kick-drift, the traversal time includes it
+ * and the kick-drift time is ~0).  kd_reset_timings zeroes the sums.  No counterpart in the
  * reference (which times whole runs, Parallel/RustVersion/src/main.rs:25-31). */
 int kd_timings(kd_ctx *ctx, double out[4]);
 int kd_reset_timings(kd_ctx *ctx);

@@ -86,6 +86,11 @@ struct kd_ctx {
   static constexpr int LAZY_PAUSE = 16;
   bool lazy_allowed = false;  // single-system context
   bool lazy_on = true;        // kd_set_tuning("lazy_dimension", 0) turns it off
+  // kd_step's walk applies the kick-drift to its own bodies (no acceleration
+  // arrays, no kick-drift pass); kd_set_tuning("fused_kick_drift", 0) keeps
+  // the separate pass.  wave_range: 6 words per wavefront of the walk.
+  bool fused_kick = true;
+  unsigned long long *wave_range = nullptr;
   int lazy_dim = 3;           // dimension the next build skips (3: none)
   int lazy_pause = 0;
   int64_t lazy_builds = 0, lazy_redos = 0;
@@ -279,6 +284,7 @@ static int create_impl(kd_ctx *c, int64_t cap, const Sizing &z) {
     ALLOC(c->allocs, c->acc[d], N);
     ALLOC(c->allocs, c->acc_o[d], N);
   }
+  ALLOC(c->allocs, c->wave_range, 6 * ((N + 63) / 64));
   c->b.sort_tmp_bytes = z.sort_bytes;
   for (int d = 0; d < 3; ++d) {
     ALLOC(c->allocs, c->b.keys_in[d], N);
@@ -601,11 +607,24 @@ int kd_step(kd_ctx *c, int64_t steps) {
     // after the first step the previous kick-drift computed the range
     if ((rc = build_impl(c, s > 0))) return rc;
     if ((rc = record(c, 1))) return rc;
-    if ((rc = traverse_impl(c))) return rc;
-    if ((rc = record(c, 2))) return rc;
-    HIP_TRY(launch_kick_drift(c->st[c->cur], c->acc[0], c->acc[1], c->acc[2], (uint32_t)c->n,
-                              c->params.dt, c->bad, c->stream,
-                              s + 1 < steps ? c->b.range : nullptr));
+    const bool more = s + 1 < steps;
+    if (c->fused_kick && !c->params.count_interactions) {
+      // the walk advances each body as it finishes it; the next build's
+      // range comes from the wavefronts' ranges
+      const uint32_t n = (uint32_t)c->n;
+      const KickDrift kd{c->st[c->cur], c->params.dt, c->bad, more ? c->wave_range : nullptr};
+      HIP_TRY(launch_traverse_kick(c->params.precision, c->traverse_variant, c->tree, c->pos64,
+                                   c->pos32, n, c->params.theta, c->params.leaf_size, kd,
+                                   c->stream));
+      if (more) HIP_TRY(launch_range_from_waves(c->wave_range, (n + 63) / 64, c->b.range, c->stream));
+      if ((rc = record(c, 2))) return rc;
+    } else {
+      if ((rc = traverse_impl(c))) return rc;
+      if ((rc = record(c, 2))) return rc;
+      HIP_TRY(launch_kick_drift(c->st[c->cur], c->acc[0], c->acc[1], c->acc[2], (uint32_t)c->n,
+                                c->params.dt, c->bad, c->stream,
+                                more ? c->b.range : nullptr));
+    }
     if ((rc = record(c, 3))) return rc;
   }
   if (c->params.profile && (rc = drain_timings(c))) return rc;
@@ -649,6 +668,10 @@ int kd_set_tuning(kd_ctx *c, const char *key, int64_t value) {
     c->lazy_pause = 0;
     return KD_OK;
   }
+  if (strcmp(key, "fused_kick_drift") == 0) {
+    c->fused_kick = value != 0;
+    return KD_OK;
+  }
   return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -666,6 +689,8 @@ int kd_get_tuning(kd_ctx *c, const char *key, int64_t *value) {
     *value = c->lazy_builds;
   } else if (strcmp(key, "lazy_redos") == 0) {
     *value = c->lazy_redos;
+  } else if (strcmp(key, "fused_kick_drift") == 0) {
+    *value = c->fused_kick ? 1 : 0;
   } else {
     return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
   }

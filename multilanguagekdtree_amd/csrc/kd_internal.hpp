@@ -263,6 +263,24 @@ hipError_t launch_traverse(int precision, bool count, int variant, const TreeDev
                            const double4 *pos64, const float4 *pos32, uint32_t n,
                            double theta, int leaf, double *ax, double *ay, double *az,
                            unsigned long long *counters, hipStream_t s);
+// The kick-drift a walk applies to its own bodies as it finishes them (the
+// step of kd_step): the tree-ordered state advanced in place (the walk reads
+// only its own body records, never this state), dt, the non-finite flag, and
+// when the next build wants the coordinate range, 6 words per wavefront that
+// receive the ordered-key range of its updated positions (lo x, lo y, lo z,
+// hi x, hi y, hi z).  The accelerations themselves are not stored.
+struct KickDrift {
+  StateDev s;
+  double dt;
+  int *bad;
+  unsigned long long *wave_range;
+};
+hipError_t launch_traverse_kick(int precision, int variant, const TreeDev &t, const double4 *pos64,
+                                const float4 *pos32, uint32_t n, double theta, int leaf,
+                                const KickDrift &kd, hipStream_t s);
+// The coordinate range (reset first) from nw wavefronts' ranges.
+hipError_t launch_range_from_waves(const unsigned long long *wave_range, uint32_t nw,
+                                   unsigned long long *range, hipStream_t s);
 // Launch shape of the forest walk: threads per block (64..512) and resident
 // wavefronts per CU for the persistent form (0: one wavefront per task).
 struct WalkShape {
