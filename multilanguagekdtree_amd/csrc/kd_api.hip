@@ -90,6 +90,9 @@ struct kd_ctx {
   // arrays, no kick-drift pass); kd_set_tuning("fused_kick_drift", 0) keeps
   // the separate pass.  wave_range: 6 words per wavefront of the walk.
   bool fused_kick = true;
+  // kd_set_tuning("mid_levels", 0): every device-wide level of the build as
+  // its own kernels (for comparison; the trees are the same)
+  bool mid_levels = true;
   unsigned long long *wave_range = nullptr;
   int lazy_dim = 3;           // dimension the next build skips (3: none)
   int lazy_pause = 0;
@@ -479,7 +482,7 @@ static int build_impl(kd_ctx *c, bool range_ready = false) {
   const int lazy = report && c->lazy_pause == 0 ? c->lazy_dim : 3;
   HIP_TRY(launch_build(c->forest, c->params.leaf_size, c->st[c->cur], c->st[nxt], c->tree, c->b,
                        c->pos64, c->pos32, c->params.theta, c->stream, range_ready, lazy,
-                       report));
+                       report, c->mid_levels));
   if (report) {
     HIP_TRY(hipEventSynchronize(c->b.ev_viol));
     uint32_t unsafe = *c->b.viol_host;
@@ -493,7 +496,7 @@ static int build_impl(kd_ctx *c, bool range_ready = false) {
         ++c->lazy_redos;
         HIP_TRY(launch_build(c->forest, c->params.leaf_size, c->st[c->cur], c->st[nxt], c->tree,
                              c->b, c->pos64, c->pos32, c->params.theta, c->stream, true, 3,
-                             true));
+                             true, c->mid_levels));
         HIP_TRY(hipEventSynchronize(c->b.ev_viol));
         unsafe = *c->b.viol_host;
         full = true;
@@ -672,6 +675,10 @@ int kd_set_tuning(kd_ctx *c, const char *key, int64_t value) {
     c->fused_kick = value != 0;
     return KD_OK;
   }
+  if (strcmp(key, "mid_levels") == 0) {
+    c->mid_levels = value != 0;
+    return KD_OK;
+  }
   return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -691,6 +698,8 @@ int kd_get_tuning(kd_ctx *c, const char *key, int64_t *value) {
     *value = c->lazy_redos;
   } else if (strcmp(key, "fused_kick_drift") == 0) {
     *value = c->fused_kick ? 1 : 0;
+  } else if (strcmp(key, "mid_levels") == 0) {
+    *value = c->mid_levels ? 1 : 0;
   } else {
     return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
   }

@@ -582,6 +582,57 @@ __device__ __forceinline__ int split_dim_of(const double e[3]) {
 // With viol set and lazy == 3, the block reports every dimension d that the
 // rule would pick with its extent raised to its global extent.
 constexpr int SPLIT_THREADS = 256;
+// The split of segment h ([s, s + c) of its lists, list-buffer mask m): the
+// node's dimension, value and size, the children's masks, and whether equal
+// keys straddle the median (the caller then orders that run).
+__device__ __forceinline__ int split_decide(const ListBufs &LB, const Coords &P, const TreeDev &t,
+                                            uint32_t h, uint32_t s, uint32_t c, uint32_t m,
+                                            const unsigned long long *range, int lazy,
+                                            uint32_t *viol, bool &tie) {
+  double ext[3], glob[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    glob[d] = ordkey_inv(range[2 * d + 1]) - ordkey_inv(range[2 * d]);
+    if (d == lazy) {
+      ext[d] = glob[d];
+      continue;
+    }
+    const uint32_t *ld = LB.in(m, d);
+    double lo = P.c[d][ld[s]];
+    double hi = P.c[d][ld[s + c - 1]];
+    ext[d] = hi - lo;
+  }
+  int sd;
+  if (lazy < 3) {
+    if (split_dim_of(ext) == lazy) atomicOr(viol, 1u << lazy);
+    ext[lazy] = -1.0;
+    sd = split_dim_of(ext);
+  } else {
+    sd = split_dim_of(ext);
+    if (viol != nullptr) {
+      uint32_t unsafe = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        double e[3] = {ext[0], ext[1], ext[2]};
+        e[d] = glob[d];
+        unsafe |= split_dim_of(e) == d ? 1u << d : 0u;
+      }
+      if (unsafe) atomicOr(viol, unsafe);
+    }
+  }
+  const uint32_t mid = s + c / 2;
+  const double kv = P.c[sd][LB.in(m, sd)[mid]];
+  const double kp = P.c[sd][LB.in(m, sd)[mid - 1]];
+  const uint8_t cm = (uint8_t)(m ^ (7u ^ (1u << sd)));
+  t.lmask[2 * h + 1] = cm;
+  t.lmask[2 * h + 2] = cm;
+  t.split_dim[h] = (uint8_t)sd;
+  t.split_val[h] = kv;
+  t.size[h] = ext[sd];
+  tie = kp == kv;
+  return sd;
+}
+
 __global__ __launch_bounds__(SPLIT_THREADS) void k_level_split(ListBufs LB, Coords P,
                                                                TreeDev t0, const SysDesc *sys,
                                                                const uint32_t *oid,
@@ -597,48 +648,9 @@ __global__ __launch_bounds__(SPLIT_THREADS) void k_level_split(ListBufs LB, Coor
   const uint32_t m = t.lmask[h];
   __shared__ int sd_s, tie_s;
   if (threadIdx.x == 0) {
-    double ext[3], glob[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      glob[d] = ordkey_inv(range[2 * d + 1]) - ordkey_inv(range[2 * d]);
-      if (d == lazy) {
-        ext[d] = glob[d];
-        continue;
-      }
-      const uint32_t *ld = LB.in(m, d);
-      double lo = P.c[d][ld[s]];
-      double hi = P.c[d][ld[s + c - 1]];
-      ext[d] = hi - lo;
-    }
-    int sd;
-    if (lazy < 3) {
-      if (split_dim_of(ext) == lazy) atomicOr(viol, 1u << lazy);
-      ext[lazy] = -1.0;
-      sd = split_dim_of(ext);
-    } else {
-      sd = split_dim_of(ext);
-      if (viol != nullptr) {
-        uint32_t unsafe = 0;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          double e[3] = {ext[0], ext[1], ext[2]};
-          e[d] = glob[d];
-          unsafe |= split_dim_of(e) == d ? 1u << d : 0u;
-        }
-        if (unsafe) atomicOr(viol, unsafe);
-      }
-    }
-    const uint32_t mid = s + c / 2;
-    const double kv = P.c[sd][LB.in(m, sd)[mid]];
-    const double kp = P.c[sd][LB.in(m, sd)[mid - 1]];
-    const uint8_t cm = (uint8_t)(m ^ (7u ^ (1u << sd)));
-    t.lmask[2 * h + 1] = cm;
-    t.lmask[2 * h + 2] = cm;
-    t.split_dim[h] = (uint8_t)sd;
-    t.split_val[h] = kv;
-    t.size[h] = ext[sd];
-    sd_s = sd;
-    tie_s = kp == kv;
+    bool tie;
+    sd_s = split_decide(LB, P, t, h, s, c, m, range, lazy, viol, tie);
+    tie_s = tie;
   }
   __syncthreads();
   if (tie_s)
@@ -827,6 +839,113 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(ListBufs LB,
       }
       before += f[i];
     }
+  }
+}
+
+// Device-wide levels [L0, L1) of a single system, one workgroup per segment
+// of level L0 (at most MID_MAX bodies): each level takes the same splits and
+// the same stable partitions as k_level_split, k_level_flags, k_level_count
+// and k_level_scatter, with the workgroup's barriers between the steps in
+// place of kernel boundaries (no tile counts, no scan pass, no bit packing:
+// the partition gathers the flag bytes of its own bodies).
+constexpr int MID_THREADS = 1024, MID_ITEMS = 4;
+constexpr uint32_t MID_TILE = MID_THREADS * MID_ITEMS;
+#ifndef KD_MID_MAX
+#define KD_MID_MAX 65536
+#endif
+constexpr uint32_t MID_MAX = KD_MID_MAX;
+// and only from a level with at least this many segments (fewer workgroups
+// leave most CUs idle, where the device-wide kernels spread every level)
+#ifndef KD_MID_MIN_LEVEL
+#define KD_MID_MIN_LEVEL 8
+#endif
+constexpr int MID_MIN_LEVEL = KD_MID_MIN_LEVEL;
+constexpr uint32_t MID_SUBS = 64;  // segments of one workgroup at its deepest level
+static_assert(MID_MAX / SUB_MAX <= MID_SUBS, "sub-segment table too small");
+
+__global__ __launch_bounds__(MID_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void k_level_mid(ListBufs LB, Coords P, TreeDev t0,
+                                                           const SysDesc *sys,
+                                                           const uint32_t *oid, uint32_t L0,
+                                                           uint32_t L1,
+                                                           const unsigned long long *range,
+                                                           int lazy, uint32_t *viol,
+                                                           uint8_t *flags) {
+  __shared__ uint32_t s_start[MID_SUBS], s_cnt[MID_SUBS];
+  __shared__ uint8_t s_sd[MID_SUBS], s_mask[MID_SUBS], s_tie[MID_SUBS];
+  __shared__ uint32_t scan_sh[MID_THREADS / 64];
+  const SysDesc S = sys[0];
+  const TreeDev t = tree_at(t0, S.node_off);
+  const uint32_t k = blockIdx.x;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t lev = L0; lev < L1; ++lev) {
+    const uint32_t nsub = 1u << (lev - L0);
+    const uint32_t hb = (1u << lev) - 1 + (k << (lev - L0));
+    // the splits, one thread per segment
+    for (uint32_t j = tid; j < nsub; j += MID_THREADS) {
+      const uint32_t h = hb + j;
+      const uint32_t s = t.seg_start[h] + S.list_off, c = t.seg_count[h];
+      const uint32_t m = t.lmask[h];
+      bool tie;
+      s_sd[j] = (uint8_t)split_decide(LB, P, t, h, s, c, m, range, lazy, viol, tie);
+      s_start[j] = s;
+      s_cnt[j] = c;
+      s_mask[j] = (uint8_t)m;
+      s_tie[j] = tie ? 1 : 0;
+    }
+    __syncthreads();
+    for (uint32_t j = 0; j < nsub; ++j) {
+      if (!s_tie[j]) continue;
+      const int sd = s_sd[j];
+      tie_fix<MID_THREADS>(LB.in(s_mask[j], sd), LB.out(s_mask[j], sd), P.c[sd], oid, s_start[j],
+                           s_cnt[j]);
+      __threadfence_block();
+      __syncthreads();
+    }
+    // left/right flag of every body, from its segment's split list
+    for (uint32_t j = 0; j < nsub; ++j) {
+      const uint32_t s = s_start[j], c = s_cnt[j], mid = s + c / 2;
+      const uint32_t *ls = LB.in(s_mask[j], s_sd[j]);
+      for (uint32_t p = s + tid; p < s + c; p += MID_THREADS) flags[ls[p]] = p < mid ? 1 : 0;
+    }
+    __threadfence_block();
+    __syncthreads();
+    // stable partition of each list that moves (not the split list, which is
+    // partitioned already, nor the lazy one), in tiles of MID_TILE positions
+    for (int d = 0; d < 3; ++d) {
+      if (d == lazy) continue;
+      for (uint32_t j = 0; j < nsub; ++j) {
+        if (d == s_sd[j]) continue;
+        const uint32_t s = s_start[j], e = s + s_cnt[j], mid = s + s_cnt[j] / 2;
+        const uint32_t *ld = LB.in(s_mask[j], d);
+        uint32_t *od = LB.out(s_mask[j], d);
+        uint32_t left = s, right = mid;  // next destinations of the tile
+        for (uint32_t base = s; base < e; base += MID_TILE) {
+          const uint32_t b0 = base + tid * MID_ITEMS;
+          uint32_t id[MID_ITEMS], f[MID_ITEMS];
+          uint32_t local = 0;
+#pragma unroll
+          for (int i = 0; i < MID_ITEMS; ++i) id[i] = ld[min(b0 + i, e - 1)];
+#pragma unroll
+          for (int i = 0; i < MID_ITEMS; ++i) {
+            f[i] = b0 + i < e ? flags[id[i]] : 0u;
+            local += f[i];
+          }
+          uint32_t tot;
+          const uint32_t ex = block_excl_sum<MID_THREADS>(local, scan_sh, tot);
+          uint32_t dl = left + ex, dr = right + (b0 - base) - ex;
+#pragma unroll
+          for (int i = 0; i < MID_ITEMS; ++i) {
+            if (b0 + i >= e) break;
+            if (f[i]) od[dl++] = id[i];
+            else od[dr++] = id[i];
+          }
+          left += tot;
+          right += min(MID_TILE, e - base) - tot;
+        }
+      }
+    }
+    __threadfence_block();
+    __syncthreads();
   }
 }
 
@@ -1703,7 +1822,8 @@ hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t
 
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
-                        double theta, hipStream_t s, bool range_ready, int lazy, bool report) {
+                        double theta, hipStream_t s, bool range_ready, int lazy, bool report,
+                        bool mid) {
   const double *coord[3] = {in.x, in.y, in.z};
   const unsigned ny = (unsigned)f.nsys;
   hipError_t e;
@@ -1765,7 +1885,19 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   LB.A = Lists{{b.L[0], b.L[1], b.L[2]}};
   LB.B = Lists{{b.Lb[0], b.Lb[1], b.Lb[2]}};
   Coords P{{in.x, in.y, in.z}};
-  for (int lev = 0; lev < f.max_sub_level; ++lev) {
+  // a single system's deeper device-wide levels (segments of at most
+  // MID_MAX bodies) run in one launch, one workgroup per segment
+  int mid0 = f.max_sub_level;
+  if (mid && f.nsys == 1) {
+    uint32_t mx = f.max_n;
+    int l = 0;
+    while (mx > MID_MAX) {
+      mx = (mx + 1) / 2;
+      ++l;
+    }
+    mid0 = std::min(std::max(l, MID_MIN_LEVEL), f.max_sub_level);
+  }
+  for (int lev = 0; lev < mid0; ++lev) {
     const uint32_t nseg = 1u << lev;
     const uint32_t T = level_tiles(f, lev);
     k_level_split<<<dim3(nseg, ny), SPLIT_THREADS, 0, s>>>(LB, P, t, f.sys, in.oid, lev,
@@ -1782,6 +1914,10 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
     k_level_scatter<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.bits,
                                                                b.cnt, lazy);
   }
+  if (mid0 < f.max_sub_level)
+    k_level_mid<<<1u << mid0, MID_THREADS, 0, s>>>(LB, P, t, f.sys, in.oid, (uint32_t)mid0,
+                                                   (uint32_t)f.max_sub_level, b.range, lazy,
+                                                   report ? b.viol : nullptr, b.flags);
   if (report) {
     e = hipMemcpyAsync(b.viol_host, b.viol, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipEventRecord(b.ev_viol, s);

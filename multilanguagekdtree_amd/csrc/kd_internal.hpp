@@ -252,13 +252,15 @@ hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t
 // nor partitioned at the device-wide levels, whose splits take its global
 // extent as the bound of every segment's, and the subtree builder sorts it
 // per segment; the tree is exact unless bit `lazy` of *b.viol is set.
+// mid: a single system's device-wide levels whose segments hold at most
+// MID_MAX bodies run as one launch of one workgroup per segment.
 // report: clear *b.viol before the levels, then copy it to b.viol_host and
 // record b.ev_viol after them (a full build reports every dimension that a
 // lazy build could not have skipped).
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
                         double theta, hipStream_t s, bool range_ready = false, int lazy = 3,
-                        bool report = false);
+                        bool report = false, bool mid = true);
 hipError_t launch_traverse(int precision, bool count, int variant, const TreeDev &t,
                            const double4 *pos64, const float4 *pos32, uint32_t n,
                            double theta, int leaf, double *ax, double *ay, double *az,
