@@ -491,9 +491,11 @@ def main():
         err = dict(median=med, p99=p99, max=mx, samples=4096)
         log(f"[bench] error {err}")
 
-    # roofline of the dominant kernel (the traversal): algorithmic FLOPs per
-    # launch = 10 T + 9 A + 18 P over all bodies, from the kernel's counters
-    flops = 10 * r["tests"] + 9 * r["m2p"] + 18 * r["p2p"]
+    # roofline of the dominant kernel (the traversal, which also applies the
+    # kick-drift to its bodies): algorithmic FLOPs per launch = 10 T + 9 A +
+    # 18 P over all bodies, from the kernel's counters, + 12 per body for the
+    # kick-drift (v += dt a; p += dt v)
+    flops = 10 * r["tests"] + 9 * r["m2p"] + 18 * r["p2p"] + 12 * n
     achieved = flops / (r["traverse_ms"] * 1e-3) / 1e12
     peak = PEAK_FP32_VALU_TFLOPS if main_name == "mixed" else PEAK_FP64_VALU_TFLOPS
     # HBM bytes per launch of the walk: FETCH_SIZE and WRITE_SIZE counter
@@ -525,8 +527,8 @@ def main():
     roofline = dict(bound="valu", achieved=round(achieved, 3), peak=peak, unit="TFLOP/s",
                     frac=round(achieved / peak, 4), traffic=traffic,
                     # the walk's kernel as rocprofv3 names it (profiles/step_kernels_*.md)
-                    kernel=("kd::k_walk<kd::F32, false, false>" if main_name == "mixed"
-                            else "kd::k_walk_f64<false>"),
+                    kernel=("kd::k_walk<kd::F32, false, false, true>" if main_name == "mixed"
+                            else "kd::k_walk_f64<false, true>"),
                     kernel_ms=round(r["traverse_ms"], 3),
                     hbm_gbps=(None if traffic is None
                               else round(traffic / (r["traverse_ms"] * 1e-3) / 1e9, 1)),

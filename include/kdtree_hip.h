@@ -173,14 +173,18 @@ int kd_counters(kd_ctx *ctx, int64_t out[5]);
  * either way; "fused_kick_drift" 0/1 (default 1) -- kd_step's walk applies
  * the kick-drift to each body as it finishes it, with no acceleration arrays
  * and no separate pass (steps with params.count_interactions set keep the
- * separate pass), bit-identical either way.  Unknown keys return
- * KD_ERR_ARG. */
+ * separate pass), bit-identical either way; "mid_levels" 0/1 (default 1) --
+ * when a single system builds from the previous tree order, its device-wide
+ * levels whose segments hold at most 65,536 bodies (from level 8 on) run as
+ * one launch of one workgroup per segment, with the same trees.  Unknown keys
+ * return KD_ERR_ARG. */
 int kd_set_tuning(kd_ctx *ctx, const char *key, int64_t value);
 
-/* Reads a knob of kd_set_tuning, or the lazy-dimension state: "lazy_next"
- * (the dimension the next build skips, 3 for none), "lazy_builds" (builds
- * that skipped one) and "lazy_redos" (of those, builds repeated in full).
- * No counterpart in the reference. */
+/* Reads a knob of kd_set_tuning, or the build's state: "lazy_next" (the
+ * dimension the next build skips, 3 for none), "lazy_builds" (builds that
+ * skipped one), "lazy_redos" (of those, builds repeated in full) and
+ * "mid_builds" (builds that ran workgroup levels).  No counterpart in the
+ * reference. */
 int kd_get_tuning(kd_ctx *ctx, const char *key, int64_t *value);
 
 /* With params.profile set, kd_step accumulates per phase the HIP-event time

@@ -1820,6 +1820,17 @@ hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t
   return hipGetLastError();
 }
 
+int mid_level(const Forest &f) {
+  if (f.nsys != 1) return f.max_sub_level;
+  uint32_t mx = f.max_n;
+  int l = 0;
+  while (mx > MID_MAX) {
+    mx = (mx + 1) / 2;
+    ++l;
+  }
+  return std::min(std::max(l, MID_MIN_LEVEL), f.max_sub_level);
+}
+
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
                         double theta, hipStream_t s, bool range_ready, int lazy, bool report,
@@ -1887,16 +1898,7 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   Coords P{{in.x, in.y, in.z}};
   // a single system's deeper device-wide levels (segments of at most
   // MID_MAX bodies) run in one launch, one workgroup per segment
-  int mid0 = f.max_sub_level;
-  if (mid && f.nsys == 1) {
-    uint32_t mx = f.max_n;
-    int l = 0;
-    while (mx > MID_MAX) {
-      mx = (mx + 1) / 2;
-      ++l;
-    }
-    mid0 = std::min(std::max(l, MID_MIN_LEVEL), f.max_sub_level);
-  }
+  const int mid0 = mid ? mid_level(f) : f.max_sub_level;
   for (int lev = 0; lev < mid0; ++lev) {
     const uint32_t nseg = 1u << lev;
     const uint32_t T = level_tiles(f, lev);

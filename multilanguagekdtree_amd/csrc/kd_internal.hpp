@@ -252,11 +252,12 @@ hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t
 // nor partitioned at the device-wide levels, whose splits take its global
 // extent as the bound of every segment's, and the subtree builder sorts it
 // per segment; the tree is exact unless bit `lazy` of *b.viol is set.
-// mid: a single system's device-wide levels whose segments hold at most
-// MID_MAX bodies run as one launch of one workgroup per segment.
+// mid: a single system's device-wide levels from mid_level(f) on run as one
+// launch of one workgroup per segment (mid_level(f) == f.max_sub_level: none).
 // report: clear *b.viol before the levels, then copy it to b.viol_host and
 // record b.ev_viol after them (a full build reports every dimension that a
 // lazy build could not have skipped).
+int mid_level(const Forest &f);
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
                         double theta, hipStream_t s, bool range_ready = false, int lazy = 3,

@@ -1,5 +1,6 @@
 """The build's deeper device-wide levels as one launch of one workgroup per
-segment (segments of at most 65,536 bodies, from level 8): the same splits
+segment (segments of at most 65,536 bodies, from level 8; builds from the
+previous tree order only): the same splits
 and stable partitions as the kernel-per-step levels
 (Parallel/RustVersion/src/array_kd_tree.rs:63-130), so trees must equal the
 oracle's and the kernel-per-step build's bit for bit.  Cases that reach those
@@ -31,7 +32,10 @@ def cube(n, seed):
     return bodies(rng.uniform(-1.0, 1.0, (n, 3)), np.full(n, 1.0 / n))
 
 
-def trees(b, mid, builds=1):
+def trees(b, mid, builds=2):
+    """The tree of the last of `builds` builds of the same positions: the
+    first builds from the upload's order (always level by level), the later
+    ones from tree order, where the workgroup levels run."""
     n = len(b["x"])
     with K.Simulation(n) as s:
         s.set_tuning("mid_levels", mid)
@@ -39,6 +43,7 @@ def trees(b, mid, builds=1):
         s.upload(b)
         for _ in range(builds):
             s.build()
+        assert s.get_tuning("mid_builds") == (builds - 1 if mid else 0)
         return s.export_tree(K.KD_LAYOUT_COMPACT)
 
 
@@ -53,11 +58,11 @@ def test_mid_levels_match_oracle(orc, name, make, n):
 
 
 def test_mid_levels_lazy_ring_equal_kernel_per_step():
-    # two builds in one context: the second skips z (the lazy dimension) in
-    # the workgroup levels too
+    # three builds in one context: the later ones skip z (the lazy dimension)
+    # in the workgroup levels too
     b = K.ring(5_000_000, seed=98)
-    a = trees(b, 1, builds=2)
-    c = trees(b, 0, builds=2)
+    a = trees(b, 1, builds=3)
+    c = trees(b, 0, builds=3)
     for k in ("is_leaf", "split_dim", "num_parts", "first", "split_val", "size", "left",
               "right", "perm", "m", "cm"):
         np.testing.assert_array_equal(getattr(a, k), getattr(c, k), err_msg=k)
@@ -71,6 +76,7 @@ def test_mid_levels_step_equal():
             s.set_tuning("mid_levels", mid)
             s.upload(b)
             s.step(3)
+            assert s.get_tuning("mid_builds") == 2 * mid
             out.append(s.download())
     for k in ("x", "y", "z", "vx", "vy", "vz"):
         np.testing.assert_array_equal(out[0][k], out[1][k], err_msg=k)
