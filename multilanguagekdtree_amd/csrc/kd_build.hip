@@ -1637,17 +1637,24 @@ k_subtree(SubArgs a) {
   }
   __threadfence_block();
   __syncthreads();
-  // Node moments, one quantity per pass (m, m x, m y, m z).  The members'
-  // terms are staged in LDS in tree order; each leaf's first thread adds its
+  // Node moments (m, m x, m y, m z).  The members' terms of one quantity at a
+  // time are staged in LDS in tree order; each leaf's first thread adds its
   // members in canonical order, and the internal nodes are summed bottom-up
   // (left + right) in LDS, one level per barrier, so no level waits on
-  // global memory.  Every sum also goes to the node arrays.  The staged terms
-  // take SUB_MAX doubles and the node sums one per heap slot of the subtree
-  // (the deepest leaf of at most SUB_MAX bodies lies above depth
-  // log2(SUB_MAX), so fewer than SUB_MAX slots).
+  // global memory.  Every sum also goes to the node arrays.  When the node
+  // sums of all four quantities fit beside the staged terms (nslot heap
+  // slots of the subtree each: the ring's segments of ~1220 bodies need 511),
+  // the four bottom-up passes share their barriers; otherwise each quantity
+  // takes its own pass with SUB_MAX slots (the deepest leaf of at most
+  // SUB_MAX bodies lies above depth log2(SUB_MAX)).
   double *stg = reinterpret_cast<double *>(smem);
-  double *nsum = stg + SUB_MAX;
   static_assert(sizeof(smem) >= 2 * SUB_MAX * sizeof(double), "moments need 16 SUB_MAX bytes");
+  double *const dsts[4] = {t.mm, t.mx, t.my, t.mz};
+  const uint32_t nslot = (2u << lam_end) - 1;
+  const bool joint = (n0 + 4 * nslot) * sizeof(double) <= sizeof(smem);
+  // node sums: quantity q's at nsum + q * qstride
+  double *nsum = joint ? stg + n0 : stg + SUB_MAX;
+  const uint32_t qstride = joint ? nslot : 0;
   for (int q = 0; q < 4; ++q) {
     const double *pc = q == 1 ? a.out.x : (q == 2 ? a.out.y : a.out.z);
     for (uint32_t i = tid; i < n0; i += NT) {
@@ -1655,7 +1662,8 @@ k_subtree(SubArgs a) {
       stg[i] = q == 0 ? bm : bm * pc[s0 + i];
     }
     __syncthreads();
-    double *dst = q == 0 ? t.mm : (q == 1 ? t.mx : (q == 2 ? t.my : t.mz));
+    double *dst = dsts[q];
+    double *ns = nsum + q * qstride;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const uint32_t j = tid * IT + it;
@@ -1666,18 +1674,22 @@ k_subtree(SubArgs a) {
       // the leaf's slot in the subtree's own heap: hl + 1 = 2^(level + lam)
       // + (k << lam) + index within depth lam
       const uint32_t lam = (31 - __clz(hl[it] + 1)) - level;
-      nsum[hl[it] + (1u << lam) - (1u << (level + lam)) - (k << lam)] = sum;
+      ns[hl[it] + (1u << lam) - (1u << (level + lam)) - (k << lam)] = sum;
     }
     __syncthreads();
+    if (joint && q < 3) continue;  // the bottom-up sums of all four follow the last
     for (int lam = lam_end - 1; lam >= 0; --lam) {
       const uint32_t hbase = (1u << (level + lam)) - 1 + (k << lam);
       for (uint32_t c = tid; c < (1u << lam); c += NT) {
         uint32_t s, cnt;
         if (!sub_node(c, lam, n0, leaf, s, cnt) || cnt <= (uint32_t)leaf) continue;
         const uint32_t rel = (1u << lam) - 1 + c;
-        const double v = nsum[2 * rel + 1] + nsum[2 * rel + 2];
-        nsum[rel] = v;
-        dst[hbase + c] = v;
+        for (int r = joint ? 0 : q; r <= q; ++r) {
+          double *nr = nsum + r * qstride;
+          const double v = nr[2 * rel + 1] + nr[2 * rel + 2];
+          nr[rel] = v;
+          dsts[r][hbase + c] = v;
+        }
       }
       __syncthreads();
     }
