@@ -224,16 +224,21 @@ __global__ __launch_bounds__(256) void k_range(const double *__restrict__ x,
 
 // Body keys span [0, PAD_KEY - 1]; in a forest, the padding positions that
 // round each system up to whole wavefronts take PAD_KEY, so they sort last
-// and never share a key with a body (their order is irrelevant: the system
-// sort moves them behind every system's bodies).
+// and never share a key with a body.
 constexpr uint32_t PAD_KEY = 0xFFFFFFFFu;
 
 // The sort key of every body position j (the values the sort carries are
-// the positions themselves, read from a counting iterator).
+// the positions themselves, read from a counting iterator).  In a forest
+// (sob set) the key is the system index in the top sysbits bits over the
+// top 32 - sysbits bits of the coordinate key, so one sort orders every
+// system's bodies by coordinate and places the systems one after another
+// (a body's key still never orders it against its system's coordinate
+// order: the coordinate part is a monotone map); with sysbits =
+// bits_for(nsys) no body key reaches PAD_KEY.
 __global__ void k_make_keys32(const double *__restrict__ p, const unsigned long long *range,
                               uint32_t *__restrict__ keys, uint32_t n, uint32_t *runs,
                               const uint32_t *__restrict__ sob,
-                              const SysDesc *__restrict__ sys) {
+                              const SysDesc *__restrict__ sys, int sysbits) {
   const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j == 0) runs[0] = 0;
   if (j >= n) return;
@@ -242,8 +247,9 @@ __global__ void k_make_keys32(const double *__restrict__ p, const unsigned long 
   const double v = 0.5 * p[j];
   uint32_t k = (uint32_t)fmin(fmax((v - lo) * scale, 0.0), 4294967294.0);
   if (sob != nullptr) {
-    const SysDesc S = sys[sob[j >> 6]];
-    if (j - S.body_off >= S.n) k = PAD_KEY;
+    const uint32_t q = sob[j >> 6];
+    const SysDesc S = sys[q];
+    k = j - S.body_off >= S.n ? PAD_KEY : (q << (32 - sysbits)) | (k >> sysbits);
   }
   keys[j] = k;
 }
@@ -442,19 +448,6 @@ __global__ __launch_bounds__(256) void k_fix_long(const uint32_t *runs,
       for (uint32_t t = threadIdx.x; t < len; t += blockDim.x) ids[st + t] = a[t];
     __syncthreads();
   }
-}
-
-// Forest: system of every id of a coordinate-sorted list (nsys for padding
-// positions), the key of the stable sort that groups the list by system.
-__global__ void k_sys_keys(const uint32_t *__restrict__ ids, const uint32_t *__restrict__ sob,
-                           const SysDesc *__restrict__ sys, uint32_t nsys,
-                           uint32_t *__restrict__ key, uint32_t n) {
-  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t id = ids[j];
-  const uint32_t q = sob[id >> 6];
-  const SysDesc S = sys[q];
-  key[j] = (id - S.body_off < S.n) ? q : nsys;
 }
 
 // ---------------------------------------------------------------------------
@@ -1794,19 +1787,12 @@ static int bits_for(uint32_t v) {
   return b;
 }
 
-size_t sort_temp_bytes(uint32_t n, int nsys) {
+size_t sort_temp_bytes(uint32_t n, int /*nsys*/) {
   size_t bytes = 0;
   (void)rocprim::radix_sort_pairs<SortCfg>(nullptr, bytes, (uint32_t *)nullptr,
                                            (uint32_t *)nullptr,
                                            rocprim::counting_iterator<uint32_t>(0u),
                                            (uint32_t *)nullptr, n, 0, 32);
-  if (nsys > 1) {
-    size_t b2 = 0;
-    (void)rocprim::radix_sort_pairs<SortCfg>(nullptr, b2, (uint32_t *)nullptr,
-                                             (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                             (uint32_t *)nullptr, n, 0, bits_for((uint32_t)nsys));
-    bytes = std::max(bytes, b2);
-  }
   return bytes;
 }
 
@@ -1856,8 +1842,8 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
     if (e != hipSuccess) return e;
   }
   // a single system sorts its n bodies; a forest sorts every body position
-  // by coordinate, then stably by system, which leaves each system's bodies
-  // in coordinate order at [list_off, list_off + n) (padding last)
+  // by (system, coordinate), which leaves each system's bodies in coordinate
+  // order at [list_off, list_off + n) (padding last)
   const uint32_t nsort = f.nsys == 1 ? f.max_n : f.total;
   if (sort_temp_bytes(nsort, f.nsys) > b.sort_tmp_bytes) return hipErrorInvalidValue;
   if (!range_ready) {
@@ -1878,26 +1864,15 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
     uint32_t *key32_out = b.keys_out[d];
     k_make_keys32<<<(nsort + 255) / 256, 256, 0, sd>>>(
         coord[d], b.range + 2 * d, key32, nsort, b.runs[d],
-        f.nsys > 1 ? f.sys_of_block : nullptr, f.sys);
+        f.nsys > 1 ? f.sys_of_block : nullptr, f.sys, bits_for((uint32_t)f.nsys));
     size_t bytes = b.sort_tmp_bytes;
-    uint32_t *by_key = f.nsys == 1 ? b.L[d] : b.Lb[d];
     e = rocprim::radix_sort_pairs<SortCfg>(b.sort_tmp[d], bytes, key32, key32_out,
-                                           rocprim::counting_iterator<uint32_t>(0u), by_key,
+                                           rocprim::counting_iterator<uint32_t>(0u), b.L[d],
                                            nsort, 0, 32, sd);
     if (e != hipSuccess) return e;
     k_fix_runs<<<(nsort + 256 * FIX_PER - 1) / (256 * FIX_PER), 256, 0, sd>>>(
-        key32_out, by_key, coord[d], nsort, b.range + 2 * d, b.runs[d]);
-    k_fix_long<<<256, 256, 0, sd>>>(b.runs[d], key32_out, nsort, by_key, b.ids_in[d], coord[d]);
-    if (f.nsys > 1) {
-      uint32_t *skey = b.keys_in[d];
-      uint32_t *skey_out = b.keys_out[d];
-      k_sys_keys<<<(nsort + 255) / 256, 256, 0, sd>>>(by_key, f.sys_of_block, f.sys,
-                                                      (uint32_t)f.nsys, skey, nsort);
-      bytes = b.sort_tmp_bytes;
-      e = rocprim::radix_sort_pairs<SortCfg>(b.sort_tmp[d], bytes, skey, skey_out, by_key, b.L[d],
-                                             nsort, 0, bits_for((uint32_t)f.nsys), sd);
-      if (e != hipSuccess) return e;
-    }
+        key32_out, b.L[d], coord[d], nsort, b.range + 2 * d, b.runs[d]);
+    k_fix_long<<<256, 256, 0, sd>>>(b.runs[d], key32_out, nsort, b.L[d], b.ids_in[d], coord[d]);
   }
   for (int i = 0; i < 2 && e == hipSuccess; ++i) {
     e = hipEventRecord(b.join[i], b.side[i]);
