@@ -263,7 +263,9 @@ int kd_batch_export_tree(kd_batch *b, int64_t sys, int32_t layout, kd_tree_view 
 /* Launch-shape knobs: "graph" 0/1 (replay the step as a HIP graph),
  * "walk_threads" 64/128/256/512 (threads per block of the walk),
  * "walk_waves_per_cu" 0..64 (0: one wavefront per task; otherwise a
- * persistent grid of that many wavefronts per CU takes tasks from a queue). */
+ * persistent grid of that many wavefronts per CU takes tasks from a queue),
+ * "fused_kick_drift" 0/1 (default 1: the walk applies the kick-drift to its
+ * bodies, as kd_set_tuning's key of that name; bit-identical either way). */
 int kd_batch_set_tuning(kd_batch *b, const char *key, int64_t value);
 /* Time the walk over the uploaded batch for every launch shape (best of reps
  * runs each) and keep the fastest: out = {threads, waves_per_cu, ms}.  The

@@ -1125,6 +1125,10 @@ struct kd_batch {
   int cus = 0;
   WalkShape ws;
   bool use_graph = true;
+  // the walk applies the kick-drift to its bodies, and the next build takes
+  // the coordinate range its wavefronts reduced (every step and upload
+  // leaves a valid range in c->b.range)
+  bool fused = true;
   hipGraph_t graph[2] = {nullptr, nullptr};  // one step from state buffer 0 / 1
   hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
@@ -1158,26 +1162,34 @@ static void batch_from_padded(const kd_batch *b, const std::vector<double> &src,
   }
 }
 
-static int batch_traverse(kd_batch *b) {
+static int batch_traverse(kd_batch *b, const KickDrift *kick = nullptr) {
   kd_ctx *c = b->c;
   return launch_traverse_forest(c->params.precision, c->params.count_interactions != 0, b->ws,
                                 c->tree, c->pos64, c->pos32, c->forest, b->tasks, b->ntasks,
                                 b->queue, b->cus, c->params.theta, c->params.leaf_size,
-                                c->acc[0], c->acc[1], c->acc[2], c->counters,
-                                c->stream) == hipSuccess
+                                c->acc[0], c->acc[1], c->acc[2], c->counters, c->stream,
+                                kick) == hipSuccess
              ? KD_OK
              : fail(KD_ERR_HIP, "forest walk launch failed: %s",
                     hipGetErrorString(hipGetLastError()));
 }
 
-// One step (build, walk, kick-drift) enqueued on the context's stream.
+// One step (build, walk, kick-drift) enqueued on the context's stream.  The
+// build takes the range the upload or the previous step left; the step
+// leaves the range of its new positions.
 static int batch_step_enqueue(kd_batch *b) {
   kd_ctx *c = b->c;
-  int rc = build_impl(c);
+  int rc = build_impl(c, true);
   if (rc) return rc;
+  if (b->fused && !c->params.count_interactions) {
+    const KickDrift kd{c->st[c->cur], c->params.dt, c->bad, c->wave_range};
+    if ((rc = batch_traverse(b, &kd))) return rc;
+    HIP_TRY(launch_range_from_waves(c->wave_range, b->ntasks, c->b.range, c->stream));
+    return KD_OK;
+  }
   if ((rc = batch_traverse(b))) return rc;
   HIP_TRY(launch_kick_drift(c->st[c->cur], c->acc[0], c->acc[1], c->acc[2], c->forest.total,
-                            c->params.dt, c->bad, c->stream));
+                            c->params.dt, c->bad, c->stream, c->b.range));
   return KD_OK;
 }
 
@@ -1362,6 +1374,8 @@ extern "C" int kd_batch_upload(kd_batch *b, const double *x, const double *y, co
     c->has_state = false;
     return fail(KD_ERR_NONFINITE, "non-finite position, velocity or mass in upload");
   }
+  // the first step's build takes this range (every step leaves the next one)
+  HIP_TRY(launch_range(s, total, c->b.range, c->stream));
   c->has_state = true;
   c->has_tree = false;
   return KD_OK;
@@ -1459,6 +1473,8 @@ extern "C" int kd_batch_set_tuning(kd_batch *b, const char *key, int64_t value) 
   } else if (k == "walk_waves_per_cu") {
     if (value < 0 || value > 64) return fail(KD_ERR_ARG, "walk_waves_per_cu must be in [0, 64]");
     b->ws.waves_per_cu = (int)value;
+  } else if (k == "fused_kick_drift") {
+    b->fused = value != 0;
   } else {
     return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
   }

@@ -222,6 +222,13 @@ __global__ __launch_bounds__(256) void k_range(const double *__restrict__ x,
   range_commit(lo, hi, sh, range);
 }
 
+hipError_t launch_range(const StateDev &st, uint32_t n, unsigned long long *range, hipStream_t s) {
+  k_range_init<<<1, 64, 0, s>>>(range);
+  k_range<<<std::max(1u, std::min<uint32_t>((n + 255) / 256, RANGE_BLOCKS)), 256, 0, s>>>(
+      st.x, st.y, st.z, n, range);
+  return hipGetLastError();
+}
+
 // Body keys span [0, PAD_KEY - 1]; in a forest, the padding positions that
 // round each system up to whole wavefronts take PAD_KEY, so they sort last
 // and never share a key with a body.

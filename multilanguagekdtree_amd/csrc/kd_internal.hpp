@@ -281,6 +281,8 @@ struct KickDrift {
 hipError_t launch_traverse_kick(int precision, int variant, const TreeDev &t, const double4 *pos64,
                                 const float4 *pos32, uint32_t n, double theta, int leaf,
                                 const KickDrift &kd, hipStream_t s);
+// The coordinate range (reset first) of the first n positions of a state.
+hipError_t launch_range(const StateDev &s, uint32_t n, unsigned long long *range, hipStream_t st);
 // The coordinate range (reset first) from nw wavefronts' ranges.
 hipError_t launch_range_from_waves(const unsigned long long *wave_range, uint32_t nw,
                                    unsigned long long *range, hipStream_t s);
@@ -295,7 +297,7 @@ hipError_t launch_traverse_forest(int precision, bool count, const WalkShape &ws
                                   const Forest &f, const uint2 *tasks, uint32_t ntasks,
                                   uint32_t *queue, int cus, double theta, int leaf, double *ax,
                                   double *ay, double *az, unsigned long long *counters,
-                                  hipStream_t s);
+                                  hipStream_t s, const KickDrift *kick = nullptr);
 // One step of the reverse sweep: the step's recorded tree and tree-order
 // bodies, and the adjoint arrays (original body order).
 struct AdjStep {

@@ -483,13 +483,13 @@ __global__ __launch_bounds__(256) void k_walk(const typename P::Rec *__restrict_
 // bodies share an L2 while the largest-first order stays balanced across
 // XCDs; otherwise wavefront w of the grid runs task w.
 
-template <class P, bool COUNT, bool PERSIST, int NT>
-__global__ __launch_bounds__(NT) void k_walk_forest(const typename P::Rec *__restrict__ nodes,
-                                                    const typename P::Body *__restrict__ bodies,
-                                                    WalkArgs<P> a, const SysDesc *__restrict__ sys,
-                                                    const uint2 *__restrict__ tasks,
-                                                    uint32_t ntasks, uint32_t *queue,
-                                                    uint32_t cl2) {
+template <class P, bool COUNT, bool PERSIST, int NT, bool KD = false>
+// (eight wavefronts per SIMD: the persistent grid is sized for 32 a CU; the
+// kick-drift's pointers would otherwise take the SGPRs of an eighth)
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_walk_forest(
+    const typename P::Rec *__restrict__ nodes, const typename P::Body *__restrict__ bodies,
+    WalkArgs<P> a, const SysDesc *__restrict__ sys, const uint2 *__restrict__ tasks,
+    uint32_t ntasks, uint32_t *queue, uint32_t cl2) {
   // the wavefront index, uniform within a wavefront (NT is a multiple of 64)
   uint32_t t = uni(((PERSIST ? blockIdx.x : xcd_block()) * NT + threadIdx.x) >> 6);
   for (;;) {
@@ -509,7 +509,19 @@ __global__ __launch_bounds__(NT) void k_walk_forest(const typename P::Rec *__res
     b.ax += S.body_off;
     b.ay += S.body_off;
     b.az += S.body_off;
-    walk_wave<P, COUNT, false>(nodes + S.node_off - 1, bodies + S.body_off, b, task.y);
+    if (KD) {
+      // the system's state; its wavefront range goes to slot t (walk_wave
+      // indexes the slots by the wavefront's first body within the system)
+      StateDev &st = b.kd.s;
+      st.x += S.body_off;
+      st.y += S.body_off;
+      st.z += S.body_off;
+      st.vx += S.body_off;
+      st.vy += S.body_off;
+      st.vz += S.body_off;
+      if (b.kd.wave_range) b.kd.wave_range += 6 * ((size_t)t - (task.y >> 6));
+    }
+    walk_wave<P, COUNT, false, KD>(nodes + S.node_off - 1, bodies + S.body_off, b, task.y);
     if (!PERSIST) return;
   }
 }
@@ -687,7 +699,7 @@ hipError_t launch_traverse_kick(int precision, int variant, const TreeDev &t, co
   return hipGetLastError();
 }
 
-template <class P, bool COUNT, bool PERSIST>
+template <class P, bool COUNT, bool PERSIST, bool KD>
 static void launch_forest_nt(const WalkArgs<P> &a, const WalkShape &ws, const SysDesc *sys,
                              const uint2 *tasks, uint32_t ntasks, uint32_t *queue, int cus,
                              hipStream_t s) {
@@ -705,7 +717,7 @@ static void launch_forest_nt(const WalkArgs<P> &a, const WalkShape &ws, const Sy
     blocks = (ntasks + wpb - 1) / wpb;
 #define KD_FOREST(NT)                                                                    \
   case NT:                                                                               \
-    k_walk_forest<P, COUNT, PERSIST, NT><<<blocks, NT, 0, s>>>(a.nodes, a.bodies, a, sys, \
+    k_walk_forest<P, COUNT, PERSIST, NT, KD><<<blocks, NT, 0, s>>>(a.nodes, a.bodies, a, sys, \
                                                               tasks, ntasks, queue, cl2); \
     break;
   switch (nt) {
@@ -719,21 +731,26 @@ static void launch_forest_nt(const WalkArgs<P> &a, const WalkShape &ws, const Sy
 }
 
 template <class P>
-static void launch_forest(const WalkArgs<P> &a, bool count, const WalkShape &ws,
+static void launch_forest(const WalkArgs<P> &a, bool count, bool kick, const WalkShape &ws,
                           const SysDesc *sys, const uint2 *tasks, uint32_t ntasks,
                           uint32_t *queue, int cus, hipStream_t s) {
   const bool persist = ws.waves_per_cu > 0;
   if (persist) k_queue_reset<<<1, 64, 0, s>>>(queue);
-  if (count) {
+  if (kick) {
     if (persist)
-      launch_forest_nt<P, true, true>(a, ws, sys, tasks, ntasks, queue, cus, s);
+      launch_forest_nt<P, false, true, true>(a, ws, sys, tasks, ntasks, queue, cus, s);
     else
-      launch_forest_nt<P, true, false>(a, ws, sys, tasks, ntasks, queue, cus, s);
+      launch_forest_nt<P, false, false, true>(a, ws, sys, tasks, ntasks, queue, cus, s);
+  } else if (count) {
+    if (persist)
+      launch_forest_nt<P, true, true, false>(a, ws, sys, tasks, ntasks, queue, cus, s);
+    else
+      launch_forest_nt<P, true, false, false>(a, ws, sys, tasks, ntasks, queue, cus, s);
   } else {
     if (persist)
-      launch_forest_nt<P, false, true>(a, ws, sys, tasks, ntasks, queue, cus, s);
+      launch_forest_nt<P, false, true, false>(a, ws, sys, tasks, ntasks, queue, cus, s);
     else
-      launch_forest_nt<P, false, false>(a, ws, sys, tasks, ntasks, queue, cus, s);
+      launch_forest_nt<P, false, false, false>(a, ws, sys, tasks, ntasks, queue, cus, s);
   }
 }
 
@@ -742,20 +759,22 @@ hipError_t launch_traverse_forest(int precision, bool count, const WalkShape &ws
                                   const Forest &f, const uint2 *tasks, uint32_t ntasks,
                                   uint32_t *queue, int cus, double theta, int leaf, double *ax,
                                   double *ay, double *az, unsigned long long *counters,
-                                  hipStream_t s) {
+                                  hipStream_t s, const KickDrift *kick) {
   if (ws.threads != 64 && ws.threads != 128 && ws.threads != 256 && ws.threads != 512)
     return hipErrorInvalidValue;
   Counters c{counters};
   const double theta2 = theta * theta;
+  const KickDrift kd = kick ? *kick : KickDrift{};
+  const bool k = kick != nullptr;
   if (precision == 0) {
-    WalkArgs<F64> a{t.rec64, pos64, 0, (uint32_t)leaf, theta2, ax, ay, az, c};
-    launch_forest(a, count, ws, f.sys, tasks, ntasks, queue, cus, s);
+    WalkArgs<F64> a{t.rec64, pos64, 0, (uint32_t)leaf, theta2, ax, ay, az, c, kd};
+    launch_forest(a, count && !k, k, ws, f.sys, tasks, ntasks, queue, cus, s);
   } else if (precision == 1) {
-    WalkArgs<F32> a{t.rec32, pos32, 0, (uint32_t)leaf, theta2, ax, ay, az, c};
-    launch_forest(a, count, ws, f.sys, tasks, ntasks, queue, cus, s);
+    WalkArgs<F32> a{t.rec32, pos32, 0, (uint32_t)leaf, theta2, ax, ay, az, c, kd};
+    launch_forest(a, count && !k, k, ws, f.sys, tasks, ntasks, queue, cus, s);
   } else {
-    WalkArgs<D64> a{t.recd, pos64, 0, (uint32_t)leaf, theta2, ax, ay, az, c};
-    launch_forest(a, count, ws, f.sys, tasks, ntasks, queue, cus, s);
+    WalkArgs<D64> a{t.recd, pos64, 0, (uint32_t)leaf, theta2, ax, ay, az, c, kd};
+    launch_forest(a, count && !k, k, ws, f.sys, tasks, ntasks, queue, cus, s);
   }
   return hipGetLastError();
 }

@@ -122,6 +122,23 @@ def test_steps_equal_single_system(bodies):
             np.testing.assert_array_equal(bt.system(got, i)[f], ref[f], err_msg=f"{i} {f}")
 
 
+@pytest.mark.parametrize("graph", [1, 0])
+def test_fused_kick_drift_equals_separate_pass(bodies, graph):
+    # the walk applying the kick-drift (and the next build taking the range
+    # its wavefronts reduce) against the separate pass, in both step forms
+    out = []
+    for fused in (1, 0):
+        with K.Batch(SIZES) as bt:
+            bt.set_tuning("graph", graph)
+            bt.set_tuning("fused_kick_drift", fused)
+            bt.upload(bodies)
+            bt.step(2)
+            bt.step(2)
+            out.append(bt.download())
+    for f in ("x", "y", "z", "vx", "vy", "vz"):
+        np.testing.assert_array_equal(out[0][f], out[1][f], err_msg=f)
+
+
 def test_autotune_keeps_results(bodies):
     with K.Batch(SIZES) as bt:
         bt.upload(bodies)
