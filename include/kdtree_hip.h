@@ -265,8 +265,16 @@ int kd_batch_export_tree(kd_batch *b, int64_t sys, int32_t layout, kd_tree_view 
  * "walk_waves_per_cu" 0..64 (0: one wavefront per task; otherwise a
  * persistent grid of that many wavefronts per CU takes tasks from a queue),
  * "fused_kick_drift" 0/1 (default 1: the walk applies the kick-drift to its
- * bodies, as kd_set_tuning's key of that name; bit-identical either way). */
+ * bodies, as kd_set_tuning's key of that name; bit-identical either way),
+ * "lazy_dimension" 0/1 (default 1: as kd_set_tuning's key, with the check
+ * read after each step -- the host waits for every step -- and a step whose
+ * skip was not exact taken again from the same state; trees and states are
+ * the same either way). */
 int kd_batch_set_tuning(kd_batch *b, const char *key, int64_t value);
+/* Reads a knob of kd_batch_set_tuning, or the lazy-dimension state
+ * ("lazy_next", "lazy_builds", "lazy_redos", as kd_get_tuning).  No
+ * counterpart in the reference. */
+int kd_batch_get_tuning(kd_batch *b, const char *key, int64_t *value);
 /* Time the walk over the uploaded batch for every launch shape (best of reps
  * runs each) and keep the fastest: out = {threads, waves_per_cu, ms}.  The
  * walk is stackless, so its block size and persistence are the shape. */

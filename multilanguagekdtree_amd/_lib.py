@@ -110,6 +110,7 @@ SIGNATURES = {
     "kd_batch_step": (C.c_int, [C.c_void_p, C.c_int64]),
     "kd_batch_export_tree": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, _P(KdTreeView)]),
     "kd_batch_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int64]),
+    "kd_batch_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, _i64]),
     "kd_batch_autotune": (C.c_int, [C.c_void_p, C.c_int64, _d]),
     "kd_batch_counters": (C.c_int, [C.c_void_p, _i64]),
     "kd_adj_create": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, _P(KdParams),

@@ -1129,16 +1129,27 @@ struct kd_batch {
   // the coordinate range its wavefronts reduced (every step and upload
   // leaves a valid range in c->b.range)
   bool fused = true;
-  hipGraph_t graph[2] = {nullptr, nullptr};  // one step from state buffer 0 / 1
-  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  // The lazy dimension, checked after the step: every step's build reports
+  // which dimensions a build could have skipped (a full build) or whether
+  // the skip was exact (a lazy one); a step whose skip was not exact is
+  // taken again from the same state with a full build.
+  bool lazy_on = true;
+  int lazy_dim = 3;
+  int lazy_pause = 0;
+  int64_t lazy_builds = 0, lazy_redos = 0;
+  // one step from state buffer 0 / 1, with lazy dimension 0, 1, 2 or none
+  hipGraph_t graph[2][4] = {};
+  hipGraphExec_t exec[2][4] = {};
 };
 
 static void drop_graphs(kd_batch *b) {
   for (int i = 0; i < 2; ++i) {
-    if (b->exec[i]) (void)hipGraphExecDestroy(b->exec[i]);
-    if (b->graph[i]) (void)hipGraphDestroy(b->graph[i]);
-    b->exec[i] = nullptr;
-    b->graph[i] = nullptr;
+    for (int l = 0; l < 4; ++l) {
+      if (b->exec[i][l]) (void)hipGraphExecDestroy(b->exec[i][l]);
+      if (b->graph[i][l]) (void)hipGraphDestroy(b->graph[i][l]);
+      b->exec[i][l] = nullptr;
+      b->graph[i][l] = nullptr;
+    }
   }
 }
 
@@ -1177,10 +1188,16 @@ static int batch_traverse(kd_batch *b, const KickDrift *kick = nullptr) {
 // One step (build, walk, kick-drift) enqueued on the context's stream.  The
 // build takes the range the upload or the previous step left; the step
 // leaves the range of its new positions.
-static int batch_step_enqueue(kd_batch *b) {
+static int batch_step_enqueue(kd_batch *b, int lazy) {
   kd_ctx *c = b->c;
-  int rc = build_impl(c, true);
-  if (rc) return rc;
+  const int nxt = c->cur ^ 1;
+  c->tree.recs = records_for(c);
+  HIP_TRY(launch_build(c->forest, c->params.leaf_size, c->st[c->cur], c->st[nxt], c->tree, c->b,
+                       c->pos64, c->pos32, c->params.theta, c->stream, true, lazy, b->lazy_on,
+                       false));
+  c->cur = nxt;
+  c->has_tree = true;
+  int rc;
   if (b->fused && !c->params.count_interactions) {
     const KickDrift kd{c->st[c->cur], c->params.dt, c->bad, c->wave_range};
     if ((rc = batch_traverse(b, &kd))) return rc;
@@ -1196,12 +1213,12 @@ static int batch_step_enqueue(kd_batch *b) {
 // Capture the step that starts from state buffer `from` into a graph.  The
 // sort library records memset nodes; each becomes a fill kernel node, and a
 // graph that still holds one is refused.
-static int batch_capture(kd_batch *b, int from) {
+static int batch_capture(kd_batch *b, int from, int lazy) {
   kd_ctx *c = b->c;
   const int saved = c->cur;
   c->cur = from;
   HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
-  int rc = batch_step_enqueue(b);
+  int rc = batch_step_enqueue(b, lazy);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   c->cur = saved;
@@ -1234,8 +1251,21 @@ static int batch_capture(kd_batch *b, int from) {
     (void)hipGraphDestroy(g);
     return fail(KD_ERR_HIP, "graph instantiate: %s", hipGetErrorString(e));
   }
-  b->graph[from] = g;
-  b->exec[from] = x;
+  b->graph[from][lazy] = g;
+  b->exec[from][lazy] = x;
+  return KD_OK;
+}
+
+// One step from the current state buffer, as a graph replay or eagerly.
+static int batch_run_step(kd_batch *b, int lazy) {
+  kd_ctx *c = b->c;
+  if (!b->use_graph) return batch_step_enqueue(b, lazy);
+  const int from = c->cur;
+  int rc;
+  if (!b->exec[from][lazy] && (rc = batch_capture(b, from, lazy))) return rc;
+  HIP_TRY(hipGraphLaunch(b->exec[from][lazy], c->stream));
+  c->cur ^= 1;
+  c->has_tree = true;
   return KD_OK;
 }
 
@@ -1437,14 +1467,39 @@ extern "C" int kd_batch_step(kd_batch *b, int64_t steps) {
     HIP_TRY(hipMemsetAsync(c->counters, 0, 8 * sizeof(unsigned long long), c->stream));
   int rc;
   for (int64_t s = 0; s < steps; ++s) {
-    if (b->use_graph) {
-      const int from = c->cur;
-      if (!b->exec[from] && (rc = batch_capture(b, from))) return rc;
-      HIP_TRY(hipGraphLaunch(b->exec[from], c->stream));
-      c->cur ^= 1;
-      c->has_tree = true;
-    } else if ((rc = batch_step_enqueue(b))) {
-      return rc;
+    const int from = c->cur;
+    const int lazy = b->lazy_on && b->lazy_pause == 0 ? b->lazy_dim : 3;
+    if ((rc = batch_run_step(b, lazy))) return rc;
+    if (!b->lazy_on) continue;
+    // the step's build report (its copy to the host is part of the step)
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    uint32_t unsafe = *c->b.viol_host;
+    bool full = lazy == 3;
+    if (lazy < 3) {
+      ++b->lazy_builds;
+      if (unsafe & (1u << lazy)) {
+        // the skipped dimension could have been split at a device-wide
+        // level: take the step again from the same state (the step left the
+        // next state's range, so the range of this one is reduced again)
+        ++b->lazy_redos;
+        c->cur = from;
+        HIP_TRY(launch_range(c->st[from], c->forest.total, c->b.range, c->stream));
+        if ((rc = batch_run_step(b, 3))) return rc;
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        unsafe = *c->b.viol_host;
+        full = true;
+        b->lazy_pause = kd_ctx::LAZY_PAUSE;
+      }
+    } else if (b->lazy_pause > 0) {
+      --b->lazy_pause;
+    }
+    if (full) {
+      b->lazy_dim = 3;
+      for (int d : {2, 0, 1})
+        if (!(unsafe & (1u << d))) {
+          b->lazy_dim = d;
+          break;
+        }
     }
   }
   int bad = 0;
@@ -1475,10 +1530,29 @@ extern "C" int kd_batch_set_tuning(kd_batch *b, const char *key, int64_t value) 
     b->ws.waves_per_cu = (int)value;
   } else if (k == "fused_kick_drift") {
     b->fused = value != 0;
+  } else if (k == "lazy_dimension") {
+    b->lazy_on = value != 0;
+    b->lazy_dim = 3;  // learnt again from the next full build
+    b->lazy_pause = 0;
   } else {
     return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
   }
   drop_graphs(b);  // the captured steps embed the launch shape
+  return KD_OK;
+}
+
+extern "C" int kd_batch_get_tuning(kd_batch *b, const char *key, int64_t *value) {
+  if (!b || !key || !value) return fail(KD_ERR_ARG, "NULL argument");
+  const std::string k = key;
+  if (k == "graph") *value = b->use_graph ? 1 : 0;
+  else if (k == "walk_threads") *value = b->ws.threads;
+  else if (k == "walk_waves_per_cu") *value = b->ws.waves_per_cu;
+  else if (k == "fused_kick_drift") *value = b->fused ? 1 : 0;
+  else if (k == "lazy_dimension") *value = b->lazy_on ? 1 : 0;
+  else if (k == "lazy_next") *value = b->lazy_on && b->lazy_pause == 0 ? b->lazy_dim : 3;
+  else if (k == "lazy_builds") *value = b->lazy_builds;
+  else if (k == "lazy_redos") *value = b->lazy_redos;
+  else return fail(KD_ERR_ARG, "unknown tuning key '%s'", key);
   return KD_OK;
 }
 

@@ -1843,7 +1843,6 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   const double *coord[3] = {in.x, in.y, in.z};
   const unsigned ny = (unsigned)f.nsys;
   hipError_t e;
-  if (f.nsys > 1) lazy = 3;  // a forest keeps every list
   if (report) {
     e = hipMemsetAsync(b.viol, 0, sizeof(uint32_t), s);
     if (e != hipSuccess) return e;
@@ -1916,7 +1915,10 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
                                                    report ? b.viol : nullptr, b.flags);
   if (report) {
     e = hipMemcpyAsync(b.viol_host, b.viol, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipEventRecord(b.ev_viol, s);
+    // (a captured build is waited for by the stream, not the event)
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (e == hipSuccess) e = hipStreamIsCapturing(s, &cs);
+    if (e == hipSuccess && cs == hipStreamCaptureStatusNone) e = hipEventRecord(b.ev_viol, s);
     if (e != hipSuccess) return e;
   }
   SubArgs a;

@@ -273,6 +273,11 @@ class Batch:
     def set_tuning(self, key: str, value: int):
         check(self._L.kd_batch_set_tuning(self._b, key.encode(), int(value)))
 
+    def get_tuning(self, key: str) -> int:
+        out = np.zeros(1, np.int64)
+        check(self._L.kd_batch_get_tuning(self._b, key.encode(), iptr(out)))
+        return int(out[0])
+
     def autotune(self, reps: int = 3) -> dict:
         out = np.zeros(3)
         check(self._L.kd_batch_autotune(self._b, reps, dptr(out)))

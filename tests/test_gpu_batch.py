@@ -139,6 +139,42 @@ def test_fused_kick_drift_equals_separate_pass(bodies, graph):
         np.testing.assert_array_equal(out[0][f], out[1][f], err_msg=f)
 
 
+def cubes(sizes, seed):
+    rng = np.random.default_rng(seed)
+    n = int(np.sum(sizes))
+    p = rng.uniform(-1.0, 1.0, (n, 3))
+    return dict(x=p[:, 0].copy(), y=p[:, 1].copy(), z=p[:, 2].copy(), vx=np.zeros(n),
+                vy=np.zeros(n), vz=np.zeros(n), m=np.full(n, 1.0 / n))
+
+
+@pytest.mark.parametrize("graph", [1, 0])
+def test_lazy_dimension_equals_full_builds(bodies, graph):
+    # thin ring systems: after the first (full) step every build skips z,
+    # and the check after each step holds; then cubes in the same batch: the
+    # first step after the upload skips z again, its check fails, and the
+    # step is taken again with a full build
+    out = []
+    for lazy in (1, 0):
+        with K.Batch(SIZES) as bt:
+            bt.set_tuning("graph", graph)
+            bt.set_tuning("lazy_dimension", lazy)
+            bt.upload(bodies)
+            bt.step(3)
+            if lazy:
+                assert bt.get_tuning("lazy_builds") == 2 and bt.get_tuning("lazy_redos") == 0
+                assert bt.get_tuning("lazy_next") == 2
+            first = bt.download()
+            bt.upload(cubes(SIZES, 7))
+            bt.step(2)
+            if lazy:
+                assert bt.get_tuning("lazy_redos") == 1
+                assert bt.get_tuning("lazy_next") == 3
+            out.append((first, bt.download()))
+    for a, b in zip(out[0], out[1]):
+        for f in ("x", "y", "z", "vx", "vy", "vz"):
+            np.testing.assert_array_equal(a[f], b[f], err_msg=f)
+
+
 def test_autotune_keeps_results(bodies):
     with K.Batch(SIZES) as bt:
         bt.upload(bodies)
