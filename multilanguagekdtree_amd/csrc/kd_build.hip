@@ -863,7 +863,8 @@ constexpr int MID_MIN_LEVEL = KD_MID_MIN_LEVEL;
 constexpr uint32_t MID_SUBS = 64;  // segments of one workgroup at its deepest level
 static_assert(MID_MAX / SUB_MAX <= MID_SUBS, "sub-segment table too small");
 
-__global__ __launch_bounds__(MID_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void k_level_mid(ListBufs LB, Coords P, TreeDev t0,
+__global__ __launch_bounds__(MID_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void
+k_level_mid(ListBufs LB, Coords P, TreeDev t0,
                                                            const SysDesc *sys,
                                                            const uint32_t *oid, uint32_t L0,
                                                            uint32_t L1,
@@ -901,11 +902,24 @@ __global__ __launch_bounds__(MID_THREADS) __attribute__((amdgpu_waves_per_eu(8))
       __threadfence_block();
       __syncthreads();
     }
-    // left/right flag of every body, from its segment's split list
+    // left/right flag of every body, from its segment's split list (four
+    // positions a thread at a time: the list loads, then the flag stores)
     for (uint32_t j = 0; j < nsub; ++j) {
-      const uint32_t s = s_start[j], c = s_cnt[j], mid = s + c / 2;
+      const uint32_t s = s_start[j], e = s + s_cnt[j], mid = s + s_cnt[j] / 2;
       const uint32_t *ls = LB.in(s_mask[j], s_sd[j]);
-      for (uint32_t p = s + tid; p < s + c; p += MID_THREADS) flags[ls[p]] = p < mid ? 1 : 0;
+      for (uint32_t p0 = s + tid; p0 < e; p0 += 4 * MID_THREADS) {
+        uint32_t id[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t p = p0 + q * MID_THREADS;
+          id[q] = ls[min(p, e - 1)];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const uint32_t p = p0 + q * MID_THREADS;
+          if (p < e) flags[id[q]] = p < mid ? 1 : 0;
+        }
+      }
     }
     __threadfence_block();
     __syncthreads();
@@ -919,28 +933,50 @@ __global__ __launch_bounds__(MID_THREADS) __attribute__((amdgpu_waves_per_eu(8))
         const uint32_t *ld = LB.in(s_mask[j], d);
         uint32_t *od = LB.out(s_mask[j], d);
         uint32_t left = s, right = mid;  // next destinations of the tile
-        for (uint32_t base = s; base < e; base += MID_TILE) {
+        // software pipelined: while tile t is scanned and scattered, the flag
+        // gathers of tile t + 1 and the list loads of tile t + 2 are in flight
+        auto load_ids = [&](uint32_t base, uint32_t (&id)[MID_ITEMS]) {
           const uint32_t b0 = base + tid * MID_ITEMS;
-          uint32_t id[MID_ITEMS], f[MID_ITEMS];
-          uint32_t local = 0;
 #pragma unroll
           for (int i = 0; i < MID_ITEMS; ++i) id[i] = ld[min(b0 + i, e - 1)];
+        };
+        auto gather = [&](uint32_t base, const uint32_t (&id)[MID_ITEMS],
+                          uint32_t (&f)[MID_ITEMS]) {
+          const uint32_t b0 = base + tid * MID_ITEMS;
 #pragma unroll
-          for (int i = 0; i < MID_ITEMS; ++i) {
-            f[i] = b0 + i < e ? flags[id[i]] : 0u;
-            local += f[i];
-          }
+          for (int i = 0; i < MID_ITEMS; ++i) f[i] = b0 + i < e ? flags[id[i]] : 0u;
+        };
+        const uint32_t ntile = (e - s + MID_TILE - 1) / MID_TILE;
+        uint32_t idc[MID_ITEMS], fc[MID_ITEMS], idn[MID_ITEMS] = {};
+        load_ids(s, idc);
+        gather(s, idc, fc);
+        if (ntile > 1) load_ids(s + MID_TILE, idn);
+        for (uint32_t t = 0; t < ntile; ++t) {
+          const uint32_t base = s + t * MID_TILE;
+          uint32_t fn[MID_ITEMS] = {}, idnn[MID_ITEMS] = {};
+          if (t + 1 < ntile) gather(base + MID_TILE, idn, fn);
+          if (t + 2 < ntile) load_ids(base + 2 * MID_TILE, idnn);
+          const uint32_t b0 = base + tid * MID_ITEMS;
+          uint32_t local = 0;
+#pragma unroll
+          for (int i = 0; i < MID_ITEMS; ++i) local += fc[i];
           uint32_t tot;
           const uint32_t ex = block_excl_sum<MID_THREADS>(local, scan_sh, tot);
           uint32_t dl = left + ex, dr = right + (b0 - base) - ex;
 #pragma unroll
           for (int i = 0; i < MID_ITEMS; ++i) {
             if (b0 + i >= e) break;
-            if (f[i]) od[dl++] = id[i];
-            else od[dr++] = id[i];
+            if (fc[i]) od[dl++] = idc[i];
+            else od[dr++] = idc[i];
           }
           left += tot;
           right += min(MID_TILE, e - base) - tot;
+#pragma unroll
+          for (int i = 0; i < MID_ITEMS; ++i) {
+            idc[i] = idn[i];
+            fc[i] = fn[i];
+            idn[i] = idnn[i];
+          }
         }
       }
     }
