@@ -174,9 +174,10 @@ int kd_counters(kd_ctx *ctx, int64_t out[5]);
  * the kick-drift to each body as it finishes it, with no acceleration arrays
  * and no separate pass (steps with params.count_interactions set keep the
  * separate pass), bit-identical either way; "mid_levels" 0/1 (default 1) --
- * when a single system builds from the previous tree order, its device-wide
- * levels whose segments hold at most 65,536 bodies (from level 8 on) run as
- * one launch of one workgroup per segment, with the same trees.  Unknown keys
+ * when a build starts from the previous tree order, its device-wide levels
+ * whose segments hold at most 65,536 bodies (for a single system from level
+ * 8 on) run as one launch of one workgroup per segment, with the same trees
+ * (batches always do so from tree order).  Unknown keys
  * return KD_ERR_ARG. */
 int kd_set_tuning(kd_ctx *ctx, const char *key, int64_t value);
 

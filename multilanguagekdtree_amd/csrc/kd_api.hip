@@ -77,6 +77,9 @@ struct kd_ctx {
   Forest forest;
   std::vector<SysDesc> hsys;
   SysDesc *dsys = nullptr;
+  SysDesc *dsys_mid = nullptr;  // dsys with sub_level = mid_level
+  uint2 *dmid = nullptr;        // (system, segment) of the workgroup levels
+  size_t dmid_cap = 0;
   uint32_t *sob = nullptr;  // system of every 64 body positions
   uint32_t extra_recs = 0;  // walk records built beyond the precision's own
   // lazy dimension (one system, never captured): the build skips the sort and
@@ -183,18 +186,32 @@ static int install_forest(kd_ctx *c) {
   f = Forest();
   f.nsys = nsys;
   f.sys = c->dsys;
+  f.sys_mid = c->dsys_mid;
+  f.mid_tasks = c->dmid;
   f.sys_of_block = c->sob;
+  std::vector<uint2> tasks;
+  set_mid_levels(c->hsys, tasks);
+  if (tasks.size() > c->dmid_cap) return fail(KD_ERR_CAPACITY, "workgroup-level table too small");
+  f.n_mid_tasks = (uint32_t)tasks.size();
+  std::vector<SysDesc> mid_sys = c->hsys;
   std::vector<uint32_t> sob;
   for (int i = 0; i < nsys; ++i) {
     const SysDesc &d = c->hsys[i];
     f.max_sub_level = std::max(f.max_sub_level, (int)d.sub_level);
+    f.max_mid_level = std::max(f.max_mid_level, (int)d.mid_level);
     f.max_n = std::max(f.max_n, d.n);
     f.max_slots = std::max(f.max_slots, d.slots);
     f.total = std::max(f.total, d.body_off + d.n);
     sob.resize((d.body_off + d.n + 63) / 64, (uint32_t)i);
+    mid_sys[i].sub_level = d.mid_level;
   }
   HIP_TRY(hipMemcpyAsync(c->dsys, c->hsys.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice,
                          c->stream));
+  HIP_TRY(hipMemcpyAsync(c->dsys_mid, mid_sys.data(), sizeof(SysDesc) * nsys,
+                         hipMemcpyHostToDevice, c->stream));
+  if (!tasks.empty())
+    HIP_TRY(hipMemcpyAsync(c->dmid, tasks.data(), sizeof(uint2) * tasks.size(),
+                           hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(c->sob, sob.data(), 4 * sob.size(), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(launch_shape(c->tree, f, c->params.leaf_size, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));  // the host tables above are temporaries
@@ -325,6 +342,9 @@ static int create_impl(kd_ctx *c, int64_t cap, const Sizing &z) {
   HIP_TRY(hipMemsetAsync(c->pos32, 0, sizeof(float4) * (N + BODY_PAD), c->stream));
   ALLOC(c->allocs, c->b.cnt, z.cnt_entries);
   ALLOC(c->allocs, c->dsys, z.nsys);
+  ALLOC(c->allocs, c->dsys_mid, z.nsys);
+  c->dmid_cap = mid_tasks_max(cap, z.nsys);
+  ALLOC(c->allocs, c->dmid, c->dmid_cap);
   ALLOC(c->allocs, c->sob, N / 64 + 1);
   ALLOC(c->allocs, c->counters, 8);
   ALLOC(c->allocs, c->bad, 4);
@@ -488,7 +508,7 @@ static int build_impl(kd_ctx *c, bool range_ready = false) {
   const bool report = c->lazy_allowed && c->lazy_on;
   const int lazy = report && c->lazy_pause == 0 ? c->lazy_dim : 3;
   const bool mid = c->mid_levels && c->tree_order;
-  if (mid && mid_level(c->forest) < c->forest.max_sub_level) ++c->mid_builds;
+  if (mid && c->forest.n_mid_tasks > 0) ++c->mid_builds;
   HIP_TRY(launch_build(c->forest, c->params.leaf_size, c->st[c->cur], c->st[nxt], c->tree, c->b,
                        c->pos64, c->pos32, c->params.theta, c->stream, range_ready, lazy,
                        report, mid));
@@ -1192,11 +1212,14 @@ static int batch_step_enqueue(kd_batch *b, int lazy) {
   kd_ctx *c = b->c;
   const int nxt = c->cur ^ 1;
   c->tree.recs = records_for(c);
+  // the workgroup levels only from tree order (a step after an upload runs
+  // eagerly with the kernel-by-kernel levels)
   HIP_TRY(launch_build(c->forest, c->params.leaf_size, c->st[c->cur], c->st[nxt], c->tree, c->b,
                        c->pos64, c->pos32, c->params.theta, c->stream, true, lazy, b->lazy_on,
-                       false));
+                       c->mid_levels && c->tree_order));
   c->cur = nxt;
   c->has_tree = true;
+  c->tree_order = true;
   int rc;
   if (b->fused && !c->params.count_interactions) {
     const KickDrift kd{c->st[c->cur], c->params.dt, c->bad, c->wave_range};
@@ -1256,10 +1279,11 @@ static int batch_capture(kd_batch *b, int from, int lazy) {
   return KD_OK;
 }
 
-// One step from the current state buffer, as a graph replay or eagerly.
+// One step from the current state buffer, as a graph replay (the graphs are
+// captured from tree order) or eagerly.
 static int batch_run_step(kd_batch *b, int lazy) {
   kd_ctx *c = b->c;
-  if (!b->use_graph) return batch_step_enqueue(b, lazy);
+  if (!b->use_graph || !c->tree_order) return batch_step_enqueue(b, lazy);
   const int from = c->cur;
   int rc;
   if (!b->exec[from][lazy] && (rc = batch_capture(b, from, lazy))) return rc;
@@ -1408,6 +1432,7 @@ extern "C" int kd_batch_upload(kd_batch *b, const double *x, const double *y, co
   HIP_TRY(launch_range(s, total, c->b.range, c->stream));
   c->has_state = true;
   c->has_tree = false;
+  c->tree_order = false;
   return KD_OK;
 }
 

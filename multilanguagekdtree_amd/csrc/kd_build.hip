@@ -842,8 +842,9 @@ __global__ __launch_bounds__(LVL_THREADS) void k_level_scatter(ListBufs LB,
   }
 }
 
-// Device-wide levels [L0, L1) of a single system, one workgroup per segment
-// of level L0 (at most MID_MAX bodies): each level takes the same splits and
+// Device-wide levels [mid_level, sub_level) of each system, one workgroup per
+// segment of level mid_level (at most MID_MAX bodies; a task lists the
+// system and the segment): each level takes the same splits and
 // the same stable partitions as k_level_split, k_level_flags, k_level_count
 // and k_level_scatter, with the workgroup's barriers between the steps in
 // place of kernel boundaries (no tile counts, no scan pass, no bit packing:
@@ -864,19 +865,17 @@ constexpr uint32_t MID_SUBS = 64;  // segments of one workgroup at its deepest l
 static_assert(MID_MAX / SUB_MAX <= MID_SUBS, "sub-segment table too small");
 
 __global__ __launch_bounds__(MID_THREADS) __attribute__((amdgpu_waves_per_eu(8))) void
-k_level_mid(ListBufs LB, Coords P, TreeDev t0,
-                                                           const SysDesc *sys,
-                                                           const uint32_t *oid, uint32_t L0,
-                                                           uint32_t L1,
-                                                           const unsigned long long *range,
-                                                           int lazy, uint32_t *viol,
-                                                           uint8_t *flags) {
+k_level_mid(ListBufs LB, Coords P, TreeDev t0, const SysDesc *sys, const uint2 *tasks,
+            const uint32_t *oid, const unsigned long long *range, int lazy, uint32_t *viol,
+            uint8_t *flags) {
   __shared__ uint32_t s_start[MID_SUBS], s_cnt[MID_SUBS];
   __shared__ uint8_t s_sd[MID_SUBS], s_mask[MID_SUBS], s_tie[MID_SUBS];
   __shared__ uint32_t scan_sh[MID_THREADS / 64];
-  const SysDesc S = sys[0];
+  // this workgroup's system and its segment at the system's mid_level
+  const uint2 task = tasks[blockIdx.x];
+  const SysDesc S = sys[task.x];
   const TreeDev t = tree_at(t0, S.node_off);
-  const uint32_t k = blockIdx.x;
+  const uint32_t k = task.y, L0 = S.mid_level, L1 = S.sub_level;
   const uint32_t tid = threadIdx.x;
   for (uint32_t lev = L0; lev < L1; ++lev) {
     const uint32_t nsub = 1u << (lev - L0);
@@ -1861,15 +1860,27 @@ hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t
   return hipGetLastError();
 }
 
-int mid_level(const Forest &f) {
-  if (f.nsys != 1) return f.max_sub_level;
-  uint32_t mx = f.max_n;
-  int l = 0;
-  while (mx > MID_MAX) {
-    mx = (mx + 1) / 2;
-    ++l;
+size_t mid_tasks_max(int64_t n, int nsys) {
+  // segments at a mid_level above MID_MIN_LEVEL hold more than MID_MAX / 2
+  // bodies; at MID_MIN_LEVEL there are 2^MID_MIN_LEVEL of them
+  return (size_t)(2 * n / MID_MAX) + (size_t)nsys + ((size_t)1 << MID_MIN_LEVEL) + 1;
+}
+
+void set_mid_levels(std::vector<SysDesc> &sys, std::vector<uint2> &tasks) {
+  tasks.clear();
+  for (size_t q = 0; q < sys.size(); ++q) {
+    SysDesc &d = sys[q];
+    uint32_t mx = d.n;
+    int l = 0;
+    while (mx > MID_MAX) {
+      mx = (mx + 1) / 2;
+      ++l;
+    }
+    if (sys.size() == 1) l = std::max(l, MID_MIN_LEVEL);
+    d.mid_level = std::min((uint32_t)l, d.sub_level);
+    if (d.mid_level < d.sub_level)
+      for (uint32_t k = 0; k < (1u << d.mid_level); ++k) tasks.push_back(make_uint2((uint32_t)q, k));
   }
-  return std::min(std::max(l, MID_MIN_LEVEL), f.max_sub_level);
 }
 
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
@@ -1925,30 +1936,33 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   LB.A = Lists{{b.L[0], b.L[1], b.L[2]}};
   LB.B = Lists{{b.Lb[0], b.Lb[1], b.Lb[2]}};
   Coords P{{in.x, in.y, in.z}};
-  // a single system's deeper device-wide levels (segments of at most
-  // MID_MAX bodies) run in one launch, one workgroup per segment
-  const int mid0 = mid ? mid_level(f) : f.max_sub_level;
-  for (int lev = 0; lev < mid0; ++lev) {
+  // every system's deeper device-wide levels (segments of at most MID_MAX
+  // bodies) run in one launch, one workgroup per segment; the kernel-by-
+  // kernel levels before them read the table whose sub_level is mid_level
+  const bool use_mid = mid && f.n_mid_tasks > 0;
+  const SysDesc *lsys = use_mid ? f.sys_mid : f.sys;
+  const int lev_end = use_mid ? f.max_mid_level : f.max_sub_level;
+  for (int lev = 0; lev < lev_end; ++lev) {
     const uint32_t nseg = 1u << lev;
     const uint32_t T = level_tiles(f, lev);
-    k_level_split<<<dim3(nseg, ny), SPLIT_THREADS, 0, s>>>(LB, P, t, f.sys, in.oid, lev,
+    k_level_split<<<dim3(nseg, ny), SPLIT_THREADS, 0, s>>>(LB, P, t, lsys, in.oid, lev,
                                                            b.range, lazy,
                                                            report ? b.viol : nullptr);
-    k_level_flags<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.flags);
+    k_level_flags<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, lsys, lev, T, b.flags);
     k_level_pack<<<(b.flag_words + 255) / 256, 256, 0, s>>>(b.flags, b.bits, b.flag_words);
-    k_level_count<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.bits,
+    k_level_count<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, lsys, lev, T, b.bits,
                                                              b.cnt, lazy);
     if (T > SCAN_SMALL_T)
-      k_level_scan<1024><<<dim3(3 * nseg, ny), 1024, 0, s>>>(b.cnt, T, f.sys, lev);
+      k_level_scan<1024><<<dim3(3 * nseg, ny), 1024, 0, s>>>(b.cnt, T, lsys, lev);
     else
-      k_level_scan<64><<<dim3(3 * nseg, ny), 64, 0, s>>>(b.cnt, T, f.sys, lev);
-    k_level_scatter<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, f.sys, lev, T, b.bits,
+      k_level_scan<64><<<dim3(3 * nseg, ny), 64, 0, s>>>(b.cnt, T, lsys, lev);
+    k_level_scatter<<<dim3(nseg * T, ny), LVL_THREADS, 0, s>>>(LB, t, lsys, lev, T, b.bits,
                                                                b.cnt, lazy);
   }
-  if (mid0 < f.max_sub_level)
-    k_level_mid<<<1u << mid0, MID_THREADS, 0, s>>>(LB, P, t, f.sys, in.oid, (uint32_t)mid0,
-                                                   (uint32_t)f.max_sub_level, b.range, lazy,
-                                                   report ? b.viol : nullptr, b.flags);
+  if (use_mid)
+    k_level_mid<<<f.n_mid_tasks, MID_THREADS, 0, s>>>(LB, P, t, f.sys, f.mid_tasks, in.oid,
+                                                      b.range, lazy, report ? b.viol : nullptr,
+                                                      b.flags);
   if (report) {
     e = hipMemcpyAsync(b.viol_host, b.viol, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     // (a captured build is waited for by the stream, not the event)

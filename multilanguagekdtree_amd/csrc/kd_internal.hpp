@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace kd {
 
 // Largest segment the one-workgroup subtree builder holds in LDS, and its
@@ -186,12 +188,19 @@ struct SysDesc {
   uint32_t sub_level;  // levels split device-wide; the subtree builder takes the next
   uint32_t levels;
   uint32_t list_off;
-  uint32_t pad0;
+  uint32_t mid_level;  // from this level on, the device-wide levels run in k_level_mid
 };
 
 struct Forest {
   int nsys = 0;
   const SysDesc *sys = nullptr;  // device copy, nsys entries
+  // the same table with sub_level replaced by mid_level (what the kernel-
+  // by-kernel levels read when the workgroup levels take the rest), the
+  // workgroup levels' (system, segment) table and the deepest mid_level
+  const SysDesc *sys_mid = nullptr;
+  const uint2 *mid_tasks = nullptr;
+  uint32_t n_mid_tasks = 0;
+  int max_mid_level = 0;
   int max_sub_level = 0;
   uint32_t max_n = 0;
   uint32_t max_slots = 0;
@@ -257,7 +266,12 @@ hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t
 // report: clear *b.viol before the levels, then copy it to b.viol_host and
 // record b.ev_viol after them (a full build reports every dimension that a
 // lazy build could not have skipped).
-int mid_level(const Forest &f);
+// Each system's mid_level (a single system's from level 8 on, so that its
+// segments fill the GPU; a forest's from the first level whose segments hold
+// at most MID_MAX bodies) and the (system, segment) table of the workgroup
+// levels.
+void set_mid_levels(std::vector<SysDesc> &sys, std::vector<uint2> &tasks);
+size_t mid_tasks_max(int64_t n, int nsys);  // bound on the table's entries
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
                         double theta, hipStream_t s, bool range_ready = false, int lazy = 3,
