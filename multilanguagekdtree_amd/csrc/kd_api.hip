@@ -80,6 +80,8 @@ struct kd_ctx {
   SysDesc *dsys_mid = nullptr;  // dsys with sub_level = mid_level
   uint2 *dmid = nullptr;        // (system, segment) of the workgroup levels
   size_t dmid_cap = 0;
+  uint2 *dsub = nullptr;        // (system, segment) of the subtree builder (forests)
+  size_t dsub_cap = 0;
   uint32_t *sob = nullptr;  // system of every 64 body positions
   uint32_t extra_recs = 0;  // walk records built beyond the precision's own
   // lazy dimension (one system, never captured): the build skips the sort and
@@ -193,6 +195,13 @@ static int install_forest(kd_ctx *c) {
   set_mid_levels(c->hsys, tasks);
   if (tasks.size() > c->dmid_cap) return fail(KD_ERR_CAPACITY, "workgroup-level table too small");
   f.n_mid_tasks = (uint32_t)tasks.size();
+  std::vector<uint2> sub;
+  if (nsys > 1) {
+    set_sub_tasks(c->hsys, sub);
+    if (sub.size() > c->dsub_cap) return fail(KD_ERR_CAPACITY, "subtree task table too small");
+  }
+  f.sub_tasks = c->dsub;
+  f.n_sub_tasks = (uint32_t)sub.size();
   std::vector<SysDesc> mid_sys = c->hsys;
   std::vector<uint32_t> sob;
   for (int i = 0; i < nsys; ++i) {
@@ -211,6 +220,9 @@ static int install_forest(kd_ctx *c) {
                          hipMemcpyHostToDevice, c->stream));
   if (!tasks.empty())
     HIP_TRY(hipMemcpyAsync(c->dmid, tasks.data(), sizeof(uint2) * tasks.size(),
+                           hipMemcpyHostToDevice, c->stream));
+  if (!sub.empty())
+    HIP_TRY(hipMemcpyAsync(c->dsub, sub.data(), sizeof(uint2) * sub.size(),
                            hipMemcpyHostToDevice, c->stream));
   HIP_TRY(hipMemcpyAsync(c->sob, sob.data(), 4 * sob.size(), hipMemcpyHostToDevice, c->stream));
   HIP_TRY(launch_shape(c->tree, f, c->params.leaf_size, c->stream));
@@ -345,6 +357,8 @@ static int create_impl(kd_ctx *c, int64_t cap, const Sizing &z) {
   ALLOC(c->allocs, c->dsys_mid, z.nsys);
   c->dmid_cap = mid_tasks_max(cap, z.nsys);
   ALLOC(c->allocs, c->dmid, c->dmid_cap);
+  c->dsub_cap = z.nsys > 1 ? sub_tasks_max(cap, z.nsys) : 1;
+  ALLOC(c->allocs, c->dsub, c->dsub_cap);
   ALLOC(c->allocs, c->sob, N / 64 + 1);
   ALLOC(c->allocs, c->counters, 8);
   ALLOC(c->allocs, c->bad, 4);

@@ -1003,6 +1003,9 @@ struct SubArgs {
   int leaf;
   double theta2;
   int lazy;  // a dimension whose list the device-wide levels did not keep (3: none)
+  // (system, segment) of every workgroup in a forest (nullptr: a single
+  // system, workgroup k takes segment k)
+  const uint2 *tasks;
 };
 
 // Heap index (1-based) of the node after the subtree of heap slot h (0-based).
@@ -1428,8 +1431,9 @@ k_subtree(SubArgs a) {
   __shared__ double shd[2 * NT / 64];
   __shared__ uint32_t lzflag;
 
-  const SysDesc S = a.sys[blockIdx.y];
-  const uint32_t k = blockIdx.x;
+  const uint2 task = a.tasks ? a.tasks[blockIdx.x] : make_uint2(0u, blockIdx.x);
+  const SysDesc S = a.sys[task.x];
+  const uint32_t k = task.y;
   const uint32_t level = S.sub_level;
   if (k >= (1u << level)) return;
   const TreeDev t = tree_at(a.t, S.node_off);
@@ -1866,6 +1870,19 @@ size_t mid_tasks_max(int64_t n, int nsys) {
   return (size_t)(2 * n / MID_MAX) + (size_t)nsys + ((size_t)1 << MID_MIN_LEVEL) + 1;
 }
 
+void set_sub_tasks(const std::vector<SysDesc> &sys, std::vector<uint2> &tasks) {
+  tasks.clear();
+  for (size_t q = 0; q < sys.size(); ++q)
+    for (uint32_t k = 0; k < (1u << sys[q].sub_level); ++k)
+      tasks.push_back(make_uint2((uint32_t)q, k));
+}
+
+size_t sub_tasks_max(int64_t n, int nsys) {
+  // a segment at sub_level holds more than SUB_MAX / 2 bodies unless its
+  // system has none split device-wide
+  return (size_t)(2 * n / SUB_MAX) + (size_t)nsys + 1;
+}
+
 void set_mid_levels(std::vector<SysDesc> &sys, std::vector<uint2> &tasks) {
   tasks.clear();
   for (size_t q = 0; q < sys.size(); ++q) {
@@ -1986,7 +2003,8 @@ hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const Sta
   a.leaf = leaf;
   a.theta2 = theta * theta;
   a.lazy = lazy;
-  k_subtree<<<dim3(1u << f.max_sub_level, ny), SUB_THREADS, 0, s>>>(a);
+  a.tasks = f.nsys > 1 ? f.sub_tasks : nullptr;
+  k_subtree<<<f.nsys > 1 ? f.n_sub_tasks : 1u << f.max_sub_level, SUB_THREADS, 0, s>>>(a);
   if (f.max_sub_level > 0) k_moments_top<<<ny, 1024, 0, s>>>(t, f.sys, theta * theta);
   return hipGetLastError();
 }

@@ -201,6 +201,9 @@ struct Forest {
   const uint2 *mid_tasks = nullptr;
   uint32_t n_mid_tasks = 0;
   int max_mid_level = 0;
+  // (system, segment) of every subtree-builder workgroup of a forest
+  const uint2 *sub_tasks = nullptr;
+  uint32_t n_sub_tasks = 0;
   int max_sub_level = 0;
   uint32_t max_n = 0;
   uint32_t max_slots = 0;
@@ -272,6 +275,9 @@ hipError_t launch_shape(const TreeDev &t, const Forest &f, int leaf, hipStream_t
 // levels.
 void set_mid_levels(std::vector<SysDesc> &sys, std::vector<uint2> &tasks);
 size_t mid_tasks_max(int64_t n, int nsys);  // bound on the table's entries
+// The (system, segment) table of a forest's subtree builder, and its bound.
+void set_sub_tasks(const std::vector<SysDesc> &sys, std::vector<uint2> &tasks);
+size_t sub_tasks_max(int64_t n, int nsys);
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
                         const TreeDev &t, BuildBufs &b, double4 *pos64, float4 *pos32,
                         double theta, hipStream_t s, bool range_ready = false, int lazy = 3,
