@@ -1898,6 +1898,10 @@ void set_mid_levels(std::vector<SysDesc> &sys, std::vector<uint2> &tasks) {
     if (d.mid_level < d.sub_level)
       for (uint32_t k = 0; k < (1u << d.mid_level); ++k) tasks.push_back(make_uint2((uint32_t)q, k));
   }
+  // largest segments first, so that the longest workgroups start first
+  std::stable_sort(tasks.begin(), tasks.end(), [&](const uint2 &a, const uint2 &b) {
+    return (sys[a.x].n >> sys[a.x].mid_level) > (sys[b.x].n >> sys[b.x].mid_level);
+  });
 }
 
 hipError_t launch_build(const Forest &f, int leaf, const StateDev &in, const StateDev &out,
