@@ -1246,13 +1246,6 @@ __device__ __forceinline__ void lz_pass(const Pol &pol, uint32_t n0, uint32_t M,
   }
 }
 
-// LDS writes of a wavefront made visible to its own lanes.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // The network over the n0 <= SUB_MAX values already in LDS; ends with a
 // barrier.  Every pass gives thread t one set of 2^G positions (a merge's
 // last stages that do not fill a pass run as the last pass shape with its

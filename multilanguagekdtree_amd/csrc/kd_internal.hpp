@@ -52,6 +52,13 @@ __device__ __forceinline__ uint32_t xcd_block() {
   return x * q + (x < r ? x : r) + i;
 }
 
+// LDS writes of a wavefront made visible to its own lanes.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Order-preserving 64-bit key of a double; -0.0 and +0.0 share one key.
 __device__ __forceinline__ uint64_t ordkey(double v) {
   uint64_t b = (uint64_t)__double_as_longlong(v);

@@ -71,6 +71,7 @@ __device__ __forceinline__ uint32_t heap_next(uint32_t h, int &depth) {
 struct F64 {
   static constexpr int CHUNK = 4;  // leaf bodies per batch of scalar loads
   static constexpr bool SELF_BY_INDEX = true;  // self pair dropped by a select
+  static constexpr bool COMPACT = false;  // leaves are never split across lanes
   using Rec = NodeRec64;
   using Body = double4;
   using Self = double4;
@@ -144,6 +145,7 @@ __device__ __forceinline__ void kahan_fold(KahanAcc &a) {
 struct F32 {
   static constexpr int CHUNK = 8;
   static constexpr bool SELF_BY_INDEX = false;
+  static constexpr bool COMPACT = true;
   using Rec = NodeRec32;
   using Body = float4;
   using Self = float4;  // x, y, z, e2
@@ -193,6 +195,7 @@ struct F32 {
 struct D64 {
   static constexpr int CHUNK = 4;
   static constexpr bool SELF_BY_INDEX = true;
+  static constexpr bool COMPACT = false;
   using Rec = NodeRecD;
   using Body = double4;
   using Self = double4;
@@ -375,12 +378,116 @@ __device__ __forceinline__ void leaf_batch(typename P::Acc &acc, const typename 
   }
 }
 
+// Lane i's value of v from the lane that the DPP control CTRL names (0 where
+// that lane is outside the row).
+template <int CTRL>
+__device__ __forceinline__ float dpp_from(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+
+// Leaf work split across the wavefront when at most 8 of its lanes take
+// part in a leaf of c bodies.  The active lanes are compacted through the
+// wavefront's 64 LDS slots: the active lane of rank o (its place among the
+// active lanes) puts its position in slot o, and lane l then computes, for
+// owner o = l / 8, the terms of leaf bodies l mod 8, l mod 8 + 8, ...:
+// ceil(c / 8) terms a lane instead of c.  The 8 lanes of an owner sum their
+// partials across lanes (DPP within a row of 16), the last of them puts the
+// sum in the owner's slot, and the owner adds it to its accumulator.  A
+// leaf's terms are thus summed per body slot and then across slots, not in
+// leaf order: this moves only the FP32 rounding of the leaf's sum.  Lanes
+// whose owner slot is not an active lane's compute terms from stale slot
+// values that no owner reads.  Body loads past the leaf (at most 7 bodies)
+// stay inside the body arrays' padding and add +0.  The terms are
+// F32::p2p's, with the self pair adding 0 through the floor e2.
+constexpr uint32_t SPLIT_MAX = 8;  // owners of a split leaf; also its bodies a step
+
+template <class P, bool COUNT>
+__device__ __forceinline__ void leaf_split(typename P::Acc &acc, const float4 &p,
+                                           const float4 *__restrict__ base, uint32_t c,
+                                           uint32_t s0, uint32_t me, uint64_t actm, bool active,
+                                           float4 *__restrict__ slot, unsigned long long &np) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t b = lane & (SPLIT_MAX - 1u);  // the lane's body slot
+  const float4 q0 = base[b];                    // loads while the positions are exchanged
+  const uint32_t rank =
+      __builtin_amdgcn_mbcnt_hi((uint32_t)(actm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)actm, 0u));
+  wave_sync();  // the previous leaf's reads of the slots come first
+  if (active) slot[rank] = p;
+  wave_sync();
+  float *mine = (float *)(slot + lane / SPLIT_MAX);
+  const float4 own = *(const float4 *)mine;
+  float dx, dy, dz;
+  auto term = [&](const float4 &q, uint32_t j) {
+    dx = own.x - q.x;
+    dy = own.y - q.y;
+    dz = own.z - q.z;
+    const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, own.w)));
+    const float ri = __builtin_amdgcn_rsqf(d2);
+    const float g = -q.w * (ri * ri * ri);
+    return j < c ? g : 0.f;
+  };
+  float g = term(q0, b);
+  float sx = g * dx, sy = g * dy, sz = g * dz;
+  for (uint32_t j0 = SPLIT_MAX; j0 < c; j0 += SPLIT_MAX) {
+    g = term(base[j0 + b], j0 + b);
+    sx += g * dx;
+    sy += g * dy;
+    sz += g * dz;
+  }
+  // the owner's 8 partials: quads by two exchanges, then the group's last
+  // lane adds the other quad (a shift of 4)
+  sx += dpp_from<0xB1>(sx);
+  sy += dpp_from<0xB1>(sy);
+  sz += dpp_from<0xB1>(sz);
+  sx += dpp_from<0x4E>(sx);
+  sy += dpp_from<0x4E>(sy);
+  sz += dpp_from<0x4E>(sz);
+  sx += dpp_from<0x114>(sx);
+  sy += dpp_from<0x114>(sy);
+  sz += dpp_from<0x114>(sz);
+  wave_sync();
+  if (b == SPLIT_MAX - 1u) {
+    mine[0] = sx;
+    mine[1] = sy;
+    mine[2] = sz;
+  }
+  wave_sync();
+  if (active) {
+    const float *t = (const float *)(slot + rank);
+    acc.p[0] += t[0];
+    acc.p[1] += t[1];
+    acc.p[2] += t[2];
+    if (COUNT) np += c - (me - s0 < c ? 1u : 0u);
+  }
+}
+
+// Lanes set in a wave-uniform mask, counted on the scalar unit.
+__device__ __forceinline__ uint32_t mask_count(uint64_t m) {
+  uint32_t k;
+  asm("s_bcnt1_i32_b64 %0, %1" : "=s"(k) : "s"(m) : "scc");
+  return k;
+}
+
+// The wavefront's 64 LDS slots for split leaves (none for policies that do
+// not split them).
+template <class P, int NT>
+__device__ __forceinline__ float4 *walk_slots() {
+  if constexpr (P::COMPACT) {
+    __shared__ float4 slots[NT];
+    return slots + (threadIdx.x & ~63u);
+  } else {
+    return nullptr;
+  }
+}
+
 // The walk of one wavefront: bodies [wbase, wbase + 64) of a tree whose
 // records start one before nodes_m1[1] and whose bodies start at bodies[0].
 template <class P, bool COUNT, bool PRED, bool KD = false>
 __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ nodes_m1,
                                           const typename P::Body *__restrict__ bodies,
-                                          const WalkArgs<P> &a, uint32_t wbase_in) {
+                                          const WalkArgs<P> &a, uint32_t wbase_in,
+                                          float4 *__restrict__ slot) {
   // wave-uniform by construction; readfirstlane lets the compiler know it
   const uint32_t wbase = uni(wbase_in);
   const uint32_t lane = threadIdx.x & 63;
@@ -413,10 +520,19 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
     if (COUNT) ++wv;
     if (r.count <= a.leaf) {
       if (COUNT) ++wl;
-      for (uint32_t b = 0; b < r.count; b += P::CHUNK)
-        leaf_batch<P, P::CHUNK, COUNT, PRED>(acc, p, bodies + (r.start + b),
-                                             min((uint32_t)P::CHUNK, r.count - b), r.start + b,
-                                             me, active, np);
+      bool split = false;
+      if constexpr (P::COMPACT) {
+        if (mask_count(actm) <= SPLIT_MAX) {
+          leaf_split<P, COUNT>(acc, p, bodies + r.start, r.count, r.start, me, actm, active, slot,
+                               np);
+          split = true;
+        }
+      }
+      if (!split)
+        for (uint32_t b = 0; b < r.count; b += P::CHUNK)
+          leaf_batch<P, P::CHUNK, COUNT, PRED>(acc, p, bodies + (r.start + b),
+                                               min((uint32_t)P::CHUNK, r.count - b),
+                                               r.start + b, me, active, np);
       P::fold(acc);  // on every lane (a fold is valid at any point): no exec-mask change
       h = r.skip;
     } else {
@@ -466,13 +582,34 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
   }
 }
 
+#ifndef KD_WALK_WAVES
+#define KD_WALK_WAVES 8
+#endif
+#ifndef KD_WALK_XCD_CHUNK
+#define KD_WALK_XCD_CHUNK 128
+#endif
+// The walk's block order: chunks of C consecutive blocks (C = 128: 32,768
+// bodies) dealt to the 8 XCDs round robin, the last partial round split into
+// one contiguous range an XCD as xcd_block does (C = 0: xcd_block itself).
+// Each XCD walks parts of the whole tree, which evens out the work of
+// regions of different density; one range an XCD left the 10 M ring's XCDs
+// 5 % apart in work and the walk 0.09 ms longer.
+__device__ __forceinline__ uint32_t walk_block() {
+  constexpr uint32_t C = KD_WALK_XCD_CHUNK;
+  if (C == 0) return xcd_block();
+  const uint32_t nb = gridDim.x, full = nb / (8u * C) * (8u * C);
+  const uint32_t x = blockIdx.x % 8u, i = blockIdx.x / 8u;
+  if (blockIdx.x < full) return (i / C) * (8u * C) + x * C + i % C;
+  const uint32_t t = nb - full, q = t / 8u, r = t % 8u, it = (blockIdx.x - full) / 8u;
+  return full + x * q + (x < r ? x : r) + it;
+}
 template <class P, bool COUNT, bool PRED, bool KD = false>
-__global__ __launch_bounds__(256) void k_walk(const typename P::Rec *__restrict__ nodes_m1,
-                                              const typename P::Body *__restrict__ bodies,
-                                              WalkArgs<P> a) {
-  const uint32_t wbase = (xcd_block() * blockDim.x + threadIdx.x) & ~63u;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(KD_WALK_WAVES))) void k_walk(
+    const typename P::Rec *__restrict__ nodes_m1, const typename P::Body *__restrict__ bodies,
+    WalkArgs<P> a) {
+  const uint32_t wbase = (walk_block() * blockDim.x + threadIdx.x) & ~63u;
   if (wbase >= a.n) return;
-  walk_wave<P, COUNT, PRED, KD>(nodes_m1, bodies, a, wbase);
+  walk_wave<P, COUNT, PRED, KD>(nodes_m1, bodies, a, wbase, walk_slots<P, 256>());
 }
 
 // Forest walk: a task is one wavefront of one system (x: system, y: first
@@ -521,7 +658,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(8))) void k_
       st.vz += S.body_off;
       if (b.kd.wave_range) b.kd.wave_range += 6 * ((size_t)t - (task.y >> 6));
     }
-    walk_wave<P, COUNT, false, KD>(nodes + S.node_off - 1, bodies + S.body_off, b, task.y);
+    walk_wave<P, COUNT, false, KD>(nodes + S.node_off - 1, bodies + S.body_off, b, task.y,
+                                   walk_slots<P, NT>());
     if (!PERSIST) return;
   }
 }

@@ -1,7 +1,7 @@
 """The mixed-precision walks (FP32 terms with compensated FP32 sums; D64 with
 FP64 differences) against FP64: the force-error bounds of the mixed step,
 scale invariance and heavy masses (the per-body floor that makes the FP32
-self pair vanish), and distinct bodies at one position."""
+self pair vanish), every leaf size, and distinct bodies at one position."""
 import numpy as np
 import pytest
 
@@ -50,6 +50,24 @@ def test_close_to_fp64_tree_walk(prec, theta):
     r = rel(accel(b, prec, theta), accel(b, K.KD_PREC_FP64, theta))
     assert np.median(r) <= MED_TOL and np.percentile(r, 99) <= P99_TOL, (
         np.median(r), np.percentile(r, 99))
+
+
+@pytest.mark.parametrize("leaf", [2, 3, 8, 9, 17, 32])
+def test_leaf_sizes_against_fp64_tree_walk(leaf):
+    # leaves that at most 8 lanes of a wavefront take part in are split
+    # across its lanes, 8 bodies a step (one step up to leaf 8, two to four
+    # beyond it); the others take one body a step: every leaf size against
+    # the FP64 walk of the same tree
+    b = K.ring(60_000, seed=7)
+    out = []
+    for prec in (K.KD_PREC_MIXED, K.KD_PREC_FP64):
+        with K.Simulation(len(b["x"]), precision=prec, leaf_size=leaf) as s:
+            s.upload(b)
+            out.append(s.accel())
+    r = rel(out[0], out[1])
+    assert np.isfinite(out[0]).all()
+    assert np.median(r) <= MED_TOL and np.percentile(r, 99) <= P99_TOL, (
+        leaf, np.median(r), np.percentile(r, 99))
 
 
 @pytest.mark.parametrize("k", [-20, 20])
