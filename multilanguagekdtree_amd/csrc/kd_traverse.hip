@@ -407,15 +407,16 @@ __device__ __forceinline__ void leaf_split(typename P::Acc &acc, const float4 &p
                                            const float4 *__restrict__ base, uint32_t c,
                                            uint32_t s0, uint32_t me, uint64_t actm, bool active,
                                            float4 *__restrict__ slot, unsigned long long &np) {
+  constexpr uint32_t B = SPLIT_MAX;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t b = lane & (SPLIT_MAX - 1u);  // the lane's body slot
+  const uint32_t b = lane & (B - 1u);  // the lane's body slot
   const float4 q0 = base[b];                    // loads while the positions are exchanged
   const uint32_t rank =
       __builtin_amdgcn_mbcnt_hi((uint32_t)(actm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)actm, 0u));
   wave_sync();  // the previous leaf's reads of the slots come first
   if (active) slot[rank] = p;
   wave_sync();
-  float *mine = (float *)(slot + lane / SPLIT_MAX);
+  float *mine = (float *)(slot + lane / B);
   const float4 own = *(const float4 *)mine;
   float dx, dy, dz;
   auto term = [&](const float4 &q, uint32_t j) {
@@ -429,7 +430,7 @@ __device__ __forceinline__ void leaf_split(typename P::Acc &acc, const float4 &p
   };
   float g = term(q0, b);
   float sx = g * dx, sy = g * dy, sz = g * dz;
-  for (uint32_t j0 = SPLIT_MAX; j0 < c; j0 += SPLIT_MAX) {
+  for (uint32_t j0 = B; j0 < c; j0 += B) {
     g = term(base[j0 + b], j0 + b);
     sx += g * dx;
     sy += g * dy;
@@ -447,7 +448,7 @@ __device__ __forceinline__ void leaf_split(typename P::Acc &acc, const float4 &p
   sy += dpp_from<0x114>(sy);
   sz += dpp_from<0x114>(sz);
   wave_sync();
-  if (b == SPLIT_MAX - 1u) {
+  if (b == B - 1u) {
     mine[0] = sx;
     mine[1] = sy;
     mine[2] = sz;
@@ -522,7 +523,8 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
       if (COUNT) ++wl;
       bool split = false;
       if constexpr (P::COMPACT) {
-        if (mask_count(actm) <= SPLIT_MAX) {
+        const uint32_t k = mask_count(actm);
+        if (k <= SPLIT_MAX) {
           leaf_split<P, COUNT>(acc, p, bodies + r.start, r.count, r.start, me, actm, active, slot,
                                np);
           split = true;
@@ -541,7 +543,15 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
       const bool ok = P::test(p, r, a.theta2, f);
       // lane masks combined on the scalar unit, then used as lane conditions
       const uint64_t okm = __builtin_amdgcn_ballot_w64(ok);
-      const uint64_t om = actm & ~okm;
+      // the lanes that open the node, and from the same SCC the next node:
+      // the first child if any lane opens, else the skip (the compiler
+      // would test om against 0 a second time)
+      uint64_t om;
+      uint32_t hn;
+      asm("s_andn2_b64 %0, %2, %3\n\ts_cselect_b32 %1, %4, %5"
+          : "=&s"(om), "=s"(hn)
+          : "s"(actm), "s"(okm), "s"(2 * h), "s"(r.skip)
+          : "scc");
       const bool take = __builtin_amdgcn_inverse_ballot_w64(actm & okm);
       const bool open = __builtin_amdgcn_inverse_ballot_w64(om);
       // wave-uniform: the far-field term is evaluated only when a lane takes it
@@ -553,7 +563,7 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
         nt += active ? 1 : 0;
         na += take ? 1 : 0;
       }
-      h = om ? 2 * h : r.skip;
+      h = hn;
     }
   } while (h != 0);
   P::fold(acc);
