@@ -596,14 +596,15 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
 #define KD_WALK_WAVES 8
 #endif
 #ifndef KD_WALK_XCD_CHUNK
-#define KD_WALK_XCD_CHUNK 128
+#define KD_WALK_XCD_CHUNK 256
 #endif
-// The walk's block order: chunks of C consecutive blocks (C = 128: 32,768
+// The walk's block order: chunks of C consecutive blocks (C = 256: 65,536
 // bodies) dealt to the 8 XCDs round robin, the last partial round split into
 // one contiguous range an XCD as xcd_block does (C = 0: xcd_block itself).
 // Each XCD walks parts of the whole tree, which evens out the work of
-// regions of different density; one range an XCD left the 10 M ring's XCDs
-// 5 % apart in work and the walk 0.09 ms longer.
+// regions of different density: with one range an XCD the 10 M ring's XCDs
+// were 5 % apart in work and the walk took 0.1 ms longer (C = 32: 0.2 ms
+// longer, the XCDs' L2s sharing less).
 __device__ __forceinline__ uint32_t walk_block() {
   constexpr uint32_t C = KD_WALK_XCD_CHUNK;
   if (C == 0) return xcd_block();
