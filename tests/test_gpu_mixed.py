@@ -70,6 +70,24 @@ def test_leaf_sizes_against_fp64_tree_walk(leaf):
         leaf, np.median(r), np.percentile(r, 99))
 
 
+@pytest.mark.parametrize("leaf", [7, 17])
+def test_interaction_counts_against_fp64_walk(leaf):
+    # the counting walk of the mixed form (a split leaf counts its bodies,
+    # less the self pair, once for every lane that takes part) against the
+    # FP64 walk of the same tree, whose counts equal the oracle's
+    # (test_gpu_parity.py): only acceptance decisions at the threshold differ
+    b = K.ring(60_000, seed=8)
+    cs = []
+    for prec in (K.KD_PREC_MIXED, K.KD_PREC_FP64):
+        with K.Simulation(len(b["x"]), precision=prec, leaf_size=leaf,
+                          count_interactions=True) as s:
+            s.upload(b)
+            s.accel()
+            cs.append(np.array(s.counters()[:3], np.int64))
+    rel = np.abs(cs[0] - cs[1]) / cs[1]
+    assert (rel <= 1e-4).all(), (leaf, cs, rel)
+
+
 @pytest.mark.parametrize("k", [-20, 20])
 def test_scale_invariance(k):
     # positions x 2^k, masses x 2^3k: accelerations x 2^k.  Power-of-two
