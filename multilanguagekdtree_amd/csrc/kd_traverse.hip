@@ -592,6 +592,10 @@ __device__ __forceinline__ void walk_wave(const typename P::Rec *__restrict__ no
   }
 }
 
+// Eight wavefronts a SIMD (at most 64 VGPRs): a first form of the split
+// leaves needed 65 registers, which left seven wavefronts a SIMD and took
+// 11.36 ms against 10.71 ms held to 64.  The walk now needs 48 (the fused
+// F32 form), so the bound only guards against such growth.
 #ifndef KD_WALK_WAVES
 #define KD_WALK_WAVES 8
 #endif
