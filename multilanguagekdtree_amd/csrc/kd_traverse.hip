@@ -398,8 +398,9 @@ __device__ __forceinline__ float dpp_from(float v) {
 // leaf order: this moves only the FP32 rounding of the leaf's sum.  Lanes
 // whose owner slot is not an active lane's compute terms from stale slot
 // values that no owner reads.  Body loads past the leaf (at most 7 bodies)
-// stay inside the body arrays' padding and add +0.  The terms are
-// F32::p2p's, with the self pair adding 0 through the floor e2.
+// stay inside the body arrays, which carry BODY_PAD spare entries past the
+// last body, and their terms are selected out.  The terms are F32::p2p's,
+// with the self pair adding 0 through the floor e2.
 constexpr uint32_t SPLIT_MAX = 8;  // owners of a split leaf; also its bodies a step
 
 template <class P, bool COUNT>
@@ -410,7 +411,7 @@ __device__ __forceinline__ void leaf_split(typename P::Acc &acc, const float4 &p
   constexpr uint32_t B = SPLIT_MAX;
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t b = lane & (B - 1u);  // the lane's body slot
-  const float4 q0 = base[b];                    // loads while the positions are exchanged
+  const float4 q0 = base[b];           // loads while the positions are exchanged
   const uint32_t rank =
       __builtin_amdgcn_mbcnt_hi((uint32_t)(actm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)actm, 0u));
   wave_sync();  // the previous leaf's reads of the slots come first
